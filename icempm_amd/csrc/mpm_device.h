@@ -1,0 +1,234 @@
+// mpm_device.h -- per-particle and per-node arithmetic of the MLS-MPM
+// substep for gfx950, in FP64.
+//
+// Every expression is written in the evaluation order of the reference
+// routine it replaces (cited per function, paths relative to the reference
+// tree) and the library is compiled with -ffp-contract=off, so results are
+// defined IEEE sequences (sqrt and division are correctly rounded on gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace icempm {
+
+// SoA row indices of the point buffer (parameters_sim.h:26-35)
+constexpr int kRowUtil = 0, kRowP = 1, kRowQ = 2, kRowJp = 3, kRowPosX = 4, kRowVelX = 6,
+              kRowFe00 = 8, kRowBp00 = 12, kPtsRows = 16, kGridRows = 3;
+constexpr double kPi = 3.14159265358979323846;   // parameters_sim.h:21
+constexpr double kSqrt2 = 1.4142135623730950;    // gpu_implementation5.cu:22 (coeff1)
+
+// Everything a kernel needs, passed by value (lands in the kernarg segment
+// and is read into SGPRs).
+struct KParams {
+    double *pts;        // 16 rows x pp doubles
+    size_t pp;          // point pitch (doubles)
+    size_t gp;          // grid pitch (doubles)
+    double *bins;       // 2 * nbins doubles
+    int *err;           // out-of-grid flag (gpu_error_indicator)
+    int nPts, gridX, gridY, gridTotal, nbins;
+    double dt, vol, mass, h, h_inv, Dinv, mu, kappa, gravity, indRsq, indV, vmax;
+    double ics, its, iss, grainVar, tanPhi, dpThr;
+};
+
+// ----------------------------------------------------------------------------
+// Signed 2x2 SVD with rotation-only U, V.  helper_math.cuh:26-46 (polar
+// step), :80-140 (symmetric eigen step and sort); givens.cuh:99-128,161-166.
+// F, U, V row-major; U = [[uc, us], [-us, uc]], likewise V.
+struct Svd2 {
+    double uc, us, vc, vs, s0, s1;
+};
+
+__device__ __forceinline__ Svd2 svd2x2(double F00, double F01, double F10, double F11) {
+    // polar step on the column-major copy {F00, F10, F01, F11}
+    double x0 = F00 + F11;
+    double x1 = F10 - F01;
+    double den = sqrt(x0 * x0 + x1 * x1);
+    double rc = 1.0, rs = 0.0;
+    if (den != 0) {
+        rc = x0 / den;
+        rs = -x1 / den;
+    }
+    // S = G * A (rows rotated); only S00, S01, S11 are needed
+    double S00 = rc * F00 - rs * F10;
+    double S01 = rc * F01 - rs * F11;
+    double S11 = rs * F01 + rc * F11;
+    double cosine, sine, s0, s1;
+    const double y2 = S01 * S01;
+    if (y2 == 0) {
+        cosine = 1;
+        sine = 0;
+        s0 = S00;
+        s1 = S11;
+    } else {
+        double tau = 0.5 * (S00 - S11);
+        double w = sqrt(tau * tau + y2);
+        double t = (tau > 0) ? S01 / (tau + w) : S01 / (tau - w);
+        cosine = 1.0 / sqrt(t * t + 1.0);
+        sine = -t * cosine;
+        double c2 = cosine * cosine;
+        double csy = 2 * cosine * sine * S01;
+        double s2 = sine * sine;
+        s0 = c2 * S00 - csy + s2 * S11;
+        s1 = s2 * S00 + csy + c2 * S11;
+    }
+    Svd2 r;
+    if (s0 < s1) {
+        r.s0 = s1;
+        r.s1 = s0;
+        r.vc = -sine;
+        r.vs = cosine;
+    } else {
+        r.s0 = s0;
+        r.s1 = s1;
+        r.vc = cosine;
+        r.vs = sine;
+    }
+    r.uc = rc * r.vc - rs * r.vs;
+    r.us = rs * r.vc + rc * r.vs;
+    return r;
+}
+
+// F = U diag(a, b) V^T for the rotation pair of an Svd2.
+__device__ __forceinline__ void usvt(const Svd2 &r, double a, double b, double &F00, double &F01,
+                                     double &F10, double &F11) {
+    // U = [[uc, us], [-us, uc]], V = [[vc, vs], [-vs, vc]]
+    double U00 = r.uc, U01 = r.us, U10 = -r.us, U11 = r.uc;
+    double V00 = r.vc, V01 = r.vs, V10 = -r.vs, V11 = r.vc;
+    F00 = (U00 * a) * V00 + (U01 * b) * V01;
+    F01 = (U00 * a) * V10 + (U01 * b) * V11;
+    F10 = (U10 * a) * V00 + (U11 * b) * V01;
+    F11 = (U10 * a) * V10 + (U11 * b) * V11;
+}
+
+// ----------------------------------------------------------------------------
+// Kirchhoff stress tau = mu/J dev(F F^T) + kappa (J^2 - 1) I
+// (KirchhoffStress_Wolper, gpu_implementation5.cu:24-34; dev :391-394)
+__device__ __forceinline__ void kirchhoff(double F00, double F01, double F10, double F11,
+                                          double mu, double kappa, double &t00, double &t01,
+                                          double &t11) {
+    double Je = F00 * F11 - F10 * F01;
+    double b00 = F00 * F00 + F01 * F01;
+    double b01 = F00 * F10 + F01 * F11;
+    double b11 = F10 * F10 + F11 * F11;
+    double half_tr = (b00 + b11) / 2;
+    double c1 = mu * (1 / Je);
+    double c2 = kappa * (Je * Je - 1.);
+    t00 = c1 * (b00 - half_tr) + c2;
+    t01 = c1 * b01;
+    t11 = c1 * (b11 - half_tr) + c2;
+}
+
+// ----------------------------------------------------------------------------
+// Trial state of the return map (ComputePQ, gpu_implementation5.cu:36-44;
+// dev_d :386-389)
+struct Trial {
+    Svd2 svd;
+    double Je, p, q, sh0, sh1, ss0, ss1;
+};
+
+__device__ __forceinline__ Trial compute_pq(double F00, double F01, double F10, double F11,
+                                            double mu, double kappa) {
+    Trial r;
+    r.svd = svd2x2(F00, F01, F10, F11);
+    r.Je = r.svd.s0 * r.svd.s1;
+    r.p = -(kappa / 2.) * (r.Je * r.Je - 1.);
+    r.ss0 = r.svd.s0 * r.svd.s0;
+    r.ss1 = r.svd.s1 * r.svd.s1;
+    double half_sum = (r.ss0 + r.ss1) / 2;
+    double c = mu / r.Je;
+    r.sh0 = c * (r.ss0 - half_sum);
+    r.sh1 = c * (r.ss1 - half_sum);
+    r.q = kSqrt2 * sqrt(r.sh0 * r.sh0 + r.sh1 * r.sh1);
+    return r;
+}
+
+// NACC failure test for intact points with per-grain strengths
+// (CheckIfPointIsInsideFailureSurface :114-141, GetParametersForGrain :97-111)
+__device__ __forceinline__ bool fails(const KParams &k, short grain, double p, double q) {
+    double var1 = 1.0 + k.grainVar * 0.05 * (-10 + grain % 21);
+    double var2 = 1.0 + k.grainVar * 0.033 * (-15 + (grain + 3) % 30);
+    double var3 = 1.0 + k.grainVar * 0.1 * (-10 + (grain + 4) % 11);
+    double pmax = k.ics * var1;
+    double pmin = -k.its * var2;
+    double qmax = k.iss * var3;
+    double beta = -pmin / pmax;
+    double a = pmax * (1 + beta);
+    double mSq = (4 * qmax * qmax * (1 + 2 * beta)) / (a * a);
+    const double pmin2 = -3e6;
+    if (p < 0) {
+        if (p < pmin2) return true;
+        double q0 = 2 * sqrt(-pmax * pmin) * qmax / (pmax - pmin);
+        double kk = -q0 / pmin2;
+        double q_limit = kk * (p - pmin2);
+        return q > q_limit;
+    }
+    double y = (1. + 2. * beta) * q * q + mSq * (p + beta * pmax) * (p - pmax);
+    return y > 0;
+}
+
+// Drucker-Prager return map for crushed points (Wolper_Drucker_Prager
+// :48-94).  Updates Fe and Jp_inv in place.
+__device__ __forceinline__ void dp_return(const KParams &k, const Trial &r, double &F00,
+                                          double &F01, double &F10, double &F11,
+                                          double &Jp_inv) {
+    if (r.p < -k.dpThr || Jp_inv < 1) {
+        double p_new = -k.dpThr;
+        double Je_new = sqrt(-2. * p_new / k.kappa + 1.);
+        double sq = sqrt(Je_new);
+        usvt(r.svd, sq, sq, F00, F01, F10, F11);
+        Jp_inv *= Je_new / r.Je;
+    } else {
+        double q_n_1;
+        if (r.p > k.ics) {
+            q_n_1 = 0;
+        } else {
+            double q_from_dp = (r.p + k.dpThr) * k.tanPhi;
+            q_n_1 = fmin(q_from_dp, k.iss);
+        }
+        if (r.q >= q_n_1) {
+            double s_norm = q_n_1 / kSqrt2;
+            double z = r.sh0 * r.sh0 + r.sh1 * r.sh1;
+            double n0 = r.sh0, n1 = r.sh1;
+            if (z > 0) {   // normalising a zero vector leaves it zero
+                double nz = sqrt(z);
+                n0 = n0 / nz;
+                n1 = n1 / nz;
+            }
+            double c = s_norm * (r.Je / k.mu);
+            double m = (r.ss0 + r.ss1) / 2;
+            usvt(r.svd, sqrt(c * n0 + m), sqrt(c * n1 + m), F00, F01, F10, F11);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Quadratic B-spline stencil along one axis (gpu_implementation5.cu:185-193)
+// Returns false for a position whose base index cannot be represented.
+__device__ __forceinline__ bool stencil(double x, double h_inv, int &base, double &fx,
+                                        double &w0, double &w1, double &w2) {
+    double bx = x * h_inv - 0.5;
+    if (!(bx > -1.0 && bx < 2147483000.0)) return false;
+    base = (int)bx;
+    fx = x * h_inv - (double)base;
+    double v0 = 1.5 - fx;
+    double v1 = fx - 1.0;
+    double v2 = fx - 0.5;
+    w0 = 0.5 * v0 * v0;
+    w1 = 0.75 - v1 * v1;
+    w2 = 0.5 * v2 * v2;
+    return true;
+}
+
+__device__ __forceinline__ double sel3(int i, double a, double b, double c) {
+    return i == 0 ? a : (i == 1 ? b : c);
+}
+
+// Cell key of a position (SnapshotManager::SortPoints key,
+// snapshotmanager.cpp:300-301, on the kernel's base index)
+__device__ __forceinline__ int cell_key(double x, double y, double hinv, int gridX) {
+    return (int)(x * hinv - 0.5) + ((int)(y * hinv - 0.5)) * gridX;
+}
+
+}  // namespace icempm
