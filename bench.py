@@ -94,6 +94,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--case", default="indent-1.6M")
     ap.add_argument("--mode", choices=["fast", "ordered"], default="fast")
+    ap.add_argument("--pass-kind", type=int, default=None,
+                    help="particle-pass kernel variant (default: the library's)")
     ap.add_argument("--timing-stride", type=int, default=8,
                     help="bracket every k-th launch of the dominant kernel with HIP events")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -113,6 +115,8 @@ def main() -> int:
     log(f"bench: {case.name} n_points={case.n_points} grid={case.grid_x}x{case.grid_y} "
         f"mode={args.mode} warmup={args.warmup} steps={args.steps}")
     m = Model(MODE_FAST if args.mode == "fast" else MODE_ORDERED)
+    if args.pass_kind is not None:
+        m.set_pass_kind(args.pass_kind)
     t0 = time.perf_counter()
     m.load_case(case)
     log(f"bench: ingest {time.perf_counter() - t0:.2f} s")

@@ -22,6 +22,7 @@ ICEMPM_E_STATE = 3
 ICEMPM_E_NODEV = 4
 MODE_FAST = 0
 MODE_ORDERED = 1
+TUNE_PASS_KIND = 0
 
 
 class IcempmError(RuntimeError):
@@ -116,7 +117,7 @@ HIP_EXPORTS = [
     "icempm_event_record_stop", "icempm_event_elapsed_ms", "icempm_sort_points",
     "icempm_sort_raw_points", "icempm_set_mode", "icempm_get_mode", "icempm_kernel_timing",
     "icempm_kernel_timing_read", "icempm_read_grid", "icempm_read_points",
-    "icempm_write_grid", "icempm_read_bins",
+    "icempm_write_grid", "icempm_read_bins", "icempm_set_tuning", "icempm_get_tuning",
 ]
 
 HOST_EXPORTS = [
@@ -174,6 +175,8 @@ def hip():
     _sig(L, "icempm_read_points", c_int, P, POINTER(c_double))
     _sig(L, "icempm_write_grid", c_int, P, POINTER(c_double))
     _sig(L, "icempm_read_bins", c_int, P, POINTER(c_double))
+    _sig(L, "icempm_set_tuning", c_int, P, c_int, c_int)
+    _sig(L, "icempm_get_tuning", c_int, P, c_int)
     _hip = L
     return L
 

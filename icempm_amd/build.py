@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 OBJ = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["api.hip", "kernels.hip", "sort.hip"]
+HIP_SOURCES = ["api.hip", "kernels.hip", "particles_exact.hip", "particles_fast.hip", "sort.hip"]
 HOST_SOURCES = ["parameters_sim.cpp", "gpu_implementation5.cpp", "model.cpp", "snapshotmanager.cpp",
                 "scene.cpp", "host_capi.cpp"]
 
@@ -81,16 +81,30 @@ def build_host() -> str:
     return HOST_LIB
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "drop_in_driver.cpp")
+DRIVER_BIN = os.path.join(ROOT, "tests", "cpp", "drop_in_driver")
+
+
+def build_driver() -> str:
+    """The C++ frontend used by the drop-in test (links the host library)."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = DRIVER_BIN + ".tmp"
+    _run([cxx, *HOST_FLAGS, "-o", tmp, DRIVER_SRC, f"-L{HERE}", "-licempm_host", "-licempm_hip",
+          "-Wl,-rpath,$ORIGIN/../../icempm_amd", "-pthread"])
+    os.replace(tmp, DRIVER_BIN)
+    return DRIVER_BIN
+
+
 def build_all() -> list[str]:
-    out = [build_hip()]
-    if any(os.path.exists(os.path.join(HOST, s)) for s in HOST_SOURCES):
-        out.append(build_host())
+    out = [build_hip(), build_host()]
+    if os.path.exists(DRIVER_SRC):
+        out.append(build_driver())
     return out
 
 
 def clean() -> None:
     shutil.rmtree(OBJ, ignore_errors=True)
-    for p in (HIP_LIB, HOST_LIB):
+    for p in (HIP_LIB, HOST_LIB, DRIVER_BIN):
         if os.path.exists(p):
             os.remove(p)
 

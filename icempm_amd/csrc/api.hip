@@ -25,6 +25,7 @@ struct icempm_ctx {
     icy::SimParams prms{};
     bool haveParams = false;
     int mode = ICEMPM_MODE_FAST;
+    int passKind = 2;     // particle-pass kernel variant (see internal.h)
 
     // device arrays
     double *pts = nullptr, *ptsTmp = nullptr;
@@ -272,6 +273,17 @@ void free_all(icempm_ctx *c) {
     c->chunksDirty = true;
 }
 
+// the particle pass of the context's mode: FMA-contracted in FAST mode,
+// exact in ORDERED mode
+hipError_t particles(icempm_ctx *c, const KParams &k, bool g2p, bool p2g, const double *gridV,
+                     double *gridOut, int recordPQ) {
+    if (c->mode == ICEMPM_MODE_ORDERED)
+        return exact::launch_particles(k, g2p, p2g, gridV, gridOut, c->chunkStart, c->nChunks,
+                                       recordPQ, c->stream, c->passKind);
+    return fast::launch_particles(k, g2p, p2g, gridV, gridOut, c->chunkStart, c->nChunks,
+                                  recordPQ, c->stream, c->passKind);
+}
+
 // the dominant kernel of a substep, optionally bracketed by timing events
 int launch_particles_timed(icempm_ctx *c, const KParams &k, bool g2p, bool p2g,
                            const double *gridV, double *gridOut, int recordPQ, int timeIt) {
@@ -293,8 +305,7 @@ int launch_particles_timed(icempm_ctx *c, const KParams &k, bool g2p, bool p2g,
         e1 = c->tev[2 * slot + 1];
         HIPCHK(c, hipEventRecord(e0, c->stream), "timing events");
     }
-    HIPCHK(c, launch_particles(k, g2p, p2g, gridV, gridOut, c->chunkStart, c->nChunks, recordPQ,
-                               c->stream), "particle pass");
+    HIPCHK(c, particles(c, k, g2p, p2g, gridV, gridOut, recordPQ), "particle pass");
     if (timeIt) HIPCHK(c, hipEventRecord(e1, c->stream), "timing events");
     return ICEMPM_OK;
 }
@@ -490,8 +501,7 @@ int icempm_p2g(icempm_ctx *c) {
         return ICEMPM_OK;
     }
     if ((rc = ensure_chunks(c))) return rc;
-    HIPCHK(c, launch_particles(k, false, true, c->gridA, c->gridA, c->chunkStart, c->nChunks, 0,
-                               c->stream), "cuda_p2g");
+    HIPCHK(c, particles(c, k, false, true, c->gridA, c->gridA, 0), "cuda_p2g");
     return ICEMPM_OK;
 }
 
@@ -516,8 +526,7 @@ int icempm_g2p(icempm_ctx *c, int recordPQ) {
     if (rc) return rc;
     if ((rc = ensure_chunks(c))) return rc;
     const KParams k = make_kparams(c);
-    HIPCHK(c, launch_particles(k, true, false, c->gridA, nullptr, c->chunkStart, c->nChunks,
-                               recordPQ ? 1 : 0, c->stream), "cuda_g2p");
+    HIPCHK(c, particles(c, k, true, false, c->gridA, nullptr, recordPQ ? 1 : 0), "cuda_g2p");
     return ICEMPM_OK;
 }
 
@@ -548,8 +557,7 @@ int icempm_run_substeps(icempm_ctx *c, int first_step, int n, double *t,
     if ((rc = ensure_chunks(c))) return rc;
     if (!ordered) {
         HIPCHK(c, launch_fill_f64(c->gridA, c->gp * kGridRows, 0.0, c->stream), "run_substeps");
-        HIPCHK(c, launch_particles(k, false, true, c->gridA, c->gridA, c->chunkStart, c->nChunks,
-                                   0, c->stream), "run_substeps");
+        HIPCHK(c, particles(c, k, false, true, c->gridA, c->gridA, 0), "run_substeps");
     }
     for (int i = 0; i < n; ++i) {
         const int s = first_step + i;
@@ -710,6 +718,26 @@ int icempm_set_mode(icempm_ctx *c, int mode) {
 }
 
 int icempm_get_mode(const icempm_ctx *c) { return c ? c->mode : -1; }
+
+int icempm_set_tuning(icempm_ctx *c, int knob, int value) {
+    if (!c) return ICEMPM_E_ARG;
+    switch (knob) {
+    case ICEMPM_TUNE_PASS_KIND:
+        if (value < 0 || value > 2) return fail(c, ICEMPM_E_ARG, "set_tuning: pass kind is 0, 1 or 2");
+        c->passKind = value;
+        return ICEMPM_OK;
+    default:
+        return fail(c, ICEMPM_E_ARG, "set_tuning: unknown knob");
+    }
+}
+
+int icempm_get_tuning(const icempm_ctx *c, int knob) {
+    if (!c) return -1;
+    switch (knob) {
+    case ICEMPM_TUNE_PASS_KIND: return c->passKind;
+    default: return -1;
+    }
+}
 
 int icempm_kernel_timing(icempm_ctx *c, int enable) {
     if (!c || enable < 0) return ICEMPM_E_ARG;

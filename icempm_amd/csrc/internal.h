@@ -20,13 +20,25 @@ constexpr int kWinY = 4;
 hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t s);
 hipError_t launch_fill_i32(int *p, size_t n, int v, hipStream_t s);
 
-// FAST-mode particle pass.  do_g2p: gather from gridV (velocities in rows
-// 1-2) and update the particles; do_p2g: scatter the (updated) particles
-// into gridOut (mass, momentum).  gridV and gridOut may be the same buffer
-// only when one of the two passes is off.
+// ---- particles_exact.hip / particles_fast.hip ---------------------------------
+// The particle pass.  do_g2p: gather from gridV (velocities in rows 1-2) and
+// update the particles; do_p2g: scatter the (updated) particles into gridOut
+// (mass, momentum).  gridV and gridOut may be the same buffer only when one
+// of the two passes is off.  exact:: is built without FMA contraction
+// (ORDERED mode), fast:: with it (FAST mode).  kind selects the kernel
+// variant: 0 workgroup windows (shared-memory gather and scatter), 1 wavefront
+// windows (direct gather, per-wavefront scatter), 2 as 1 with the stencil
+// order rotated per lane.
+namespace exact {
 hipError_t launch_particles(const KParams &k, bool do_g2p, bool do_p2g, const double *gridV,
                             double *gridOut, const int *chunkStart, int nChunks, int recordPQ,
-                            hipStream_t s);
+                            hipStream_t s, int kind);
+}
+namespace fast {
+hipError_t launch_particles(const KParams &k, bool do_g2p, bool do_p2g, const double *gridV,
+                            double *gridOut, const int *chunkStart, int nChunks, int recordPQ,
+                            hipStream_t s, int kind);
+}
 
 // Grid update.  ordered: write each node's contact force and bin to
 // nodeBin/nodeF for a later ordered fold; otherwise accumulate into k.bins

@@ -1,15 +1,9 @@
-// kernels.hip -- MLS-MPM substep kernels for gfx950 (CDNA4), FP64.
+// kernels.hip -- grid kernels of the MLS-MPM substep for gfx950 (CDNA4),
+// FP64, built without FMA contraction.  (The particle pass is in
+// particles.inc, built twice: particles_exact.hip and particles_fast.hip.)
 //
-// FAST path (default):
-//   k_particles<G2P, P2G>  one workgroup per chunk of <= 256 cell-sorted
-//       particles.  G2P gathers grid velocities through a shared-memory
-//       window that the workgroup loads once (coalesced), P2G accumulates the
-//       chunk's 27 contributions per particle into a shared-memory window with
-//       LDS FP64 atomics and then adds each touched node to the global grid
-//       with one hardware FP64 atomic per component.  With both passes on,
-//       the kernel is the fused "G2P of substep k + P2G of substep k+1".
-//   k_update<false>  grid update with the indenter force reduced per
-//       workgroup in shared memory; also clears the next substep's grid.
+//   k_update<false>  grid update (FAST mode) with the indenter force reduced
+//       per workgroup in shared memory; also clears the next substep's grid.
 // ORDERED path (bit-reproducible):
 //   k_p2g_contrib / k_p2g_fill / k_p2g_fold  every node sums its
 //       contributions in increasing particle index;
@@ -56,277 +50,6 @@ __global__ void k_hello(int *flag) {
 
 hipError_t launch_hello(int *flag, hipStream_t s) {
     hipLaunchKernelGGL(k_hello, dim3(1), dim3(64), 0, s, flag);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int wave_min(int v) {
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-// Minimum over the workgroup; every thread gets the result.
-__device__ __forceinline__ int block_min(int v, int *red) {
-    v = wave_min(v);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) red[w] = v;
-    __syncthreads();
-    int r = red[0];
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int i = 1; i < nw; ++i) r = min(r, red[i]);
-    return r;
-}
-
-__device__ __forceinline__ void flag_error(const KParams &k) { k.err[0] = 1; }
-
-// ---------------------------------------------------------------------------
-// Particle pass.  G2P: v2_kernel_g2p (gpu_implementation5.cu:294-379);
-// P2G: v2_kernel_p2g (gpu_implementation5.cu:148-213).
-template <bool G2P, bool P2G>
-__global__ __launch_bounds__(kChunk) void k_particles(KParams k, const double *__restrict__ gridV,
-                                                      double *__restrict__ gridOut,
-                                                      const int *__restrict__ chunkStart,
-                                                      int nChunks, int recordPQ, int xcdRemap) {
-    __shared__ double win[3][kWinY * kWinX];
-    __shared__ int red[kChunk / 64];
-
-    int c = blockIdx.x;
-    if (xcdRemap) {
-        // consecutive chunks (neighbouring cells) on the same XCD: workgroups
-        // are dealt to the 8 XCDs round-robin
-        const int per = (nChunks + 7) >> 3;
-        c = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    }
-    if (c >= nChunks) return;   // whole workgroup leaves together
-    const int beg = chunkStart[c], end = chunkStart[c + 1];
-    const int pt = beg + (int)threadIdx.x;
-    const bool valid = pt < end;
-    double *__restrict__ P = k.pts;
-    const size_t pp = k.pp, gp = k.gp;
-    const int gx = k.gridX, gy = k.gridY;
-
-    double px = 0, py = 0, vx = 0, vy = 0;
-    double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
-    double B00 = 0, B01 = 0, B10 = 0, B11 = 0;
-
-    if constexpr (G2P) {
-        unsigned char *util = reinterpret_cast<unsigned char *>(P);
-        const short *grains = reinterpret_cast<const short *>(util + pp);
-        double Jp = 1;
-        unsigned char crushed = 0;
-        short grain = 0;
-        if (valid) {
-            px = P[pp * kRowPosX + pt];
-            py = P[pp * (kRowPosX + 1) + pt];
-            F00 = P[pp * (kRowFe00 + 0) + pt];
-            F01 = P[pp * (kRowFe00 + 1) + pt];
-            F10 = P[pp * (kRowFe00 + 2) + pt];
-            F11 = P[pp * (kRowFe00 + 3) + pt];
-            Jp = P[pp * kRowJp + pt];
-            crushed = util[pt];
-            grain = grains[pt];
-        }
-        int bx = 0, by = 0;
-        double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
-        bool ok = valid && stencil(px, k.h_inv, bx, fx0, wx0, wx1, wx2) &&
-                  stencil(py, k.h_inv, by, fx1, wy0, wy1, wy2);
-        const int ox = block_min(ok ? bx : INT_MAX, red);
-        const int oy = block_min(ok ? by : INT_MAX, red);
-        if (ox != INT_MAX) {
-            for (int e = threadIdx.x; e < kWinX * kWinY; e += blockDim.x) {
-                const int X = ox + e % kWinX, Y = oy + e / kWinX;
-                double a = 0, b = 0;
-                if (X < gx && Y < gy) {
-                    const int node = X + Y * gx;
-                    a = gridV[gp + node];
-                    b = gridV[2 * gp + node];
-                }
-                win[0][e] = a;
-                win[1][e] = b;
-            }
-        }
-        __syncthreads();
-        if (valid && !ok) flag_error(k);
-        if (ok) {
-            double v0 = 0, v1 = 0, b00 = 0, b01 = 0, b10 = 0, b11 = 0;
-            const double c4 = 4. * k.h_inv;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double d0 = i - fx0;
-                    const double d1 = j - fx1;
-                    const double w = sel3(i, wx0, wx1, wx2) * sel3(j, wy0, wy1, wy2);
-                    const int X = bx + i, Y = by + j;
-                    double n0 = 0, n1 = 0;
-                    if (X < 0 || Y < 0 || X >= gx || Y >= gy) {
-                        flag_error(k);
-                    } else {
-                        const int lx = X - ox, ly = Y - oy;
-                        if (lx < kWinX && ly < kWinY) {
-                            n0 = win[0][ly * kWinX + lx];
-                            n1 = win[1][ly * kWinX + lx];
-                        } else {
-                            const int node = X + Y * gx;
-                            n0 = gridV[gp + node];
-                            n1 = gridV[2 * gp + node];
-                        }
-                    }
-                    v0 += w * n0;
-                    v1 += w * n1;
-                    const double cw = c4 * w;
-                    b00 += cw * (n0 * d0);
-                    b01 += cw * (n0 * d1);
-                    b10 += cw * (n1 * d0);
-                    b11 += cw * (n1 * d1);
-                }
-            }
-            const double dt = k.dt;
-            px += v0 * dt;
-            py += v1 * dt;
-            const double M00 = 1.0 + dt * b00, M01 = dt * b01, M10 = dt * b10,
-                         M11 = 1.0 + dt * b11;
-            const double n00 = M00 * F00 + M01 * F10;
-            const double n01 = M00 * F01 + M01 * F11;
-            const double n10 = M10 * F00 + M11 * F10;
-            const double n11 = M10 * F01 + M11 * F11;
-            F00 = n00; F01 = n01; F10 = n10; F11 = n11;
-            const Trial r = compute_pq(F00, F01, F10, F11, k.mu, k.kappa);
-            if (crushed == 0) crushed = fails(k, grain, r.p, r.q) ? 1 : 0;
-            if (crushed == 1) dp_return(k, r, F00, F01, F10, F11, Jp);
-            vx = v0; vy = v1;
-            B00 = b00; B01 = b01; B10 = b10; B11 = b11;
-
-            P[pp * kRowJp + pt] = Jp;
-            P[pp * kRowPosX + pt] = px;
-            P[pp * (kRowPosX + 1) + pt] = py;
-            P[pp * kRowVelX + pt] = vx;
-            P[pp * (kRowVelX + 1) + pt] = vy;
-            P[pp * (kRowFe00 + 0) + pt] = F00;
-            P[pp * (kRowFe00 + 1) + pt] = F01;
-            P[pp * (kRowFe00 + 2) + pt] = F10;
-            P[pp * (kRowFe00 + 3) + pt] = F11;
-            P[pp * (kRowBp00 + 0) + pt] = B00;
-            P[pp * (kRowBp00 + 1) + pt] = B01;
-            P[pp * (kRowBp00 + 2) + pt] = B10;
-            P[pp * (kRowBp00 + 3) + pt] = B11;
-            util[pt] = crushed;
-            if (recordPQ) {
-                P[pp * kRowP + pt] = r.p;
-                P[pp * kRowQ + pt] = r.q;
-            }
-        }
-    }
-
-    if constexpr (P2G) {
-        bool have = valid;
-        if constexpr (!G2P) {
-            if (valid) {
-                px = P[pp * kRowPosX + pt];
-                py = P[pp * (kRowPosX + 1) + pt];
-                vx = P[pp * kRowVelX + pt];
-                vy = P[pp * (kRowVelX + 1) + pt];
-                F00 = P[pp * (kRowFe00 + 0) + pt];
-                F01 = P[pp * (kRowFe00 + 1) + pt];
-                F10 = P[pp * (kRowFe00 + 2) + pt];
-                F11 = P[pp * (kRowFe00 + 3) + pt];
-                B00 = P[pp * (kRowBp00 + 0) + pt];
-                B01 = P[pp * (kRowBp00 + 1) + pt];
-                B10 = P[pp * (kRowBp00 + 2) + pt];
-                B11 = P[pp * (kRowBp00 + 3) + pt];
-            }
-        } else {
-            // a particle whose G2P failed keeps its old state; its P2G is
-            // skipped (the run is already flagged)
-            have = valid && !(px != px || py != py);
-        }
-        int bx = 0, by = 0;
-        double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
-        const bool ok = have && stencil(px, k.h_inv, bx, fx0, wx0, wx1, wx2) &&
-                        stencil(py, k.h_inv, by, fx1, wy0, wy1, wy2);
-        if (valid && !ok) flag_error(k);
-        const int ox = block_min(ok ? bx : INT_MAX, red);
-        const int oy = block_min(ok ? by : INT_MAX, red);
-        for (int e = threadIdx.x; e < kWinX * kWinY; e += blockDim.x) {
-            win[0][e] = 0;
-            win[1][e] = 0;
-            win[2][e] = 0;
-        }
-        __syncthreads();
-        if (ok) {
-            double t00, t01, t11;
-            kirchhoff(F00, F01, F10, F11, k.mu, k.kappa, t00, t01, t11);
-            const double m = k.mass, h = k.h;
-            const double sc = k.dt * k.vol * k.Dinv;
-            const double A00 = m * B00 - sc * t00;
-            const double A01 = m * B01 - sc * t01;
-            const double A10 = m * B10 - sc * t01;
-            const double A11 = m * B11 - sc * t11;
-            const double mv0 = vx * m, mv1 = vy * m;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double Wip = sel3(i, wx0, wx1, wx2) * sel3(j, wy0, wy1, wy2);
-                    const double d0 = (i - fx0) * h;
-                    const double d1 = (j - fx1) * h;
-                    const double incV0 = Wip * (mv0 + (A00 * d0 + A01 * d1));
-                    const double incV1 = Wip * (mv1 + (A10 * d0 + A11 * d1));
-                    const double incM = Wip * m;
-                    const int X = bx + i, Y = by + j;
-                    if (X < 0 || Y < 0 || X >= gx || Y >= gy) {
-                        flag_error(k);
-                        continue;
-                    }
-                    const int lx = X - ox, ly = Y - oy;
-                    if (lx < kWinX && ly < kWinY) {
-                        const int e = ly * kWinX + lx;
-                        atomicAdd(&win[0][e], incM);
-                        atomicAdd(&win[1][e], incV0);
-                        atomicAdd(&win[2][e], incV1);
-                    } else {
-                        const int node = X + Y * gx;
-                        unsafeAtomicAdd(&gridOut[node], incM);
-                        unsafeAtomicAdd(&gridOut[gp + node], incV0);
-                        unsafeAtomicAdd(&gridOut[2 * gp + node], incV1);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        if (ox != INT_MAX) {
-            for (int e = threadIdx.x; e < kWinX * kWinY; e += blockDim.x) {
-                const double a = win[0][e];
-                if (a != 0) {
-                    const int X = ox + e % kWinX, Y = oy + e / kWinX;
-                    if (X < gx && Y < gy) {
-                        const int node = X + Y * gx;
-                        unsafeAtomicAdd(&gridOut[node], a);
-                        unsafeAtomicAdd(&gridOut[gp + node], win[1][e]);
-                        unsafeAtomicAdd(&gridOut[2 * gp + node], win[2][e]);
-                    }
-                }
-            }
-        }
-    }
-}
-
-hipError_t launch_particles(const KParams &k, bool do_g2p, bool do_p2g, const double *gridV,
-                            double *gridOut, const int *chunkStart, int nChunks, int recordPQ,
-                            hipStream_t s) {
-    if (nChunks <= 0) return hipSuccess;
-    const int xcd = 1;
-    const unsigned nb = (unsigned)(((nChunks + 7) / 8) * 8);
-    if (do_g2p && do_p2g)
-        hipLaunchKernelGGL((k_particles<true, true>), dim3(nb), dim3(kChunk), 0, s, k, gridV,
-                           gridOut, chunkStart, nChunks, recordPQ, xcd);
-    else if (do_g2p)
-        hipLaunchKernelGGL((k_particles<true, false>), dim3(nb), dim3(kChunk), 0, s, k, gridV,
-                           gridOut, chunkStart, nChunks, recordPQ, xcd);
-    else if (do_p2g)
-        hipLaunchKernelGGL((k_particles<false, true>), dim3(nb), dim3(kChunk), 0, s, k, gridV,
-                           gridOut, chunkStart, nChunks, recordPQ, xcd);
     return hipGetLastError();
 }
 

@@ -3,8 +3,10 @@
 //
 // Every expression is written in the evaluation order of the reference
 // routine it replaces (cited per function, paths relative to the reference
-// tree) and the library is compiled with -ffp-contract=off, so results are
-// defined IEEE sequences (sqrt and division are correctly rounded on gfx950).
+// tree).  Translation units built with -ffp-contract=off get defined IEEE
+// sequences (sqrt and division are correctly rounded on gfx950); the FAST
+// particle pass includes this header after `#pragma clang fp contract(fast)`.
+// All functions have internal linkage, so the two builds never collide.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,7 +42,7 @@ struct Svd2 {
     double uc, us, vc, vs, s0, s1;
 };
 
-__device__ __forceinline__ Svd2 svd2x2(double F00, double F01, double F10, double F11) {
+static __device__ __forceinline__ Svd2 svd2x2(double F00, double F01, double F10, double F11) {
     // polar step on the column-major copy {F00, F10, F01, F11}
     double x0 = F00 + F11;
     double x1 = F10 - F01;
@@ -91,7 +93,7 @@ __device__ __forceinline__ Svd2 svd2x2(double F00, double F01, double F10, doubl
 }
 
 // F = U diag(a, b) V^T for the rotation pair of an Svd2.
-__device__ __forceinline__ void usvt(const Svd2 &r, double a, double b, double &F00, double &F01,
+static __device__ __forceinline__ void usvt(const Svd2 &r, double a, double b, double &F00, double &F01,
                                      double &F10, double &F11) {
     // U = [[uc, us], [-us, uc]], V = [[vc, vs], [-vs, vc]]
     double U00 = r.uc, U01 = r.us, U10 = -r.us, U11 = r.uc;
@@ -105,7 +107,7 @@ __device__ __forceinline__ void usvt(const Svd2 &r, double a, double b, double &
 // ----------------------------------------------------------------------------
 // Kirchhoff stress tau = mu/J dev(F F^T) + kappa (J^2 - 1) I
 // (KirchhoffStress_Wolper, gpu_implementation5.cu:24-34; dev :391-394)
-__device__ __forceinline__ void kirchhoff(double F00, double F01, double F10, double F11,
+static __device__ __forceinline__ void kirchhoff(double F00, double F01, double F10, double F11,
                                           double mu, double kappa, double &t00, double &t01,
                                           double &t11) {
     double Je = F00 * F11 - F10 * F01;
@@ -128,7 +130,7 @@ struct Trial {
     double Je, p, q, sh0, sh1, ss0, ss1;
 };
 
-__device__ __forceinline__ Trial compute_pq(double F00, double F01, double F10, double F11,
+static __device__ __forceinline__ Trial compute_pq(double F00, double F01, double F10, double F11,
                                             double mu, double kappa) {
     Trial r;
     r.svd = svd2x2(F00, F01, F10, F11);
@@ -146,7 +148,7 @@ __device__ __forceinline__ Trial compute_pq(double F00, double F01, double F10, 
 
 // NACC failure test for intact points with per-grain strengths
 // (CheckIfPointIsInsideFailureSurface :114-141, GetParametersForGrain :97-111)
-__device__ __forceinline__ bool fails(const KParams &k, short grain, double p, double q) {
+static __device__ __forceinline__ bool fails(const KParams &k, short grain, double p, double q) {
     double var1 = 1.0 + k.grainVar * 0.05 * (-10 + grain % 21);
     double var2 = 1.0 + k.grainVar * 0.033 * (-15 + (grain + 3) % 30);
     double var3 = 1.0 + k.grainVar * 0.1 * (-10 + (grain + 4) % 11);
@@ -170,7 +172,7 @@ __device__ __forceinline__ bool fails(const KParams &k, short grain, double p, d
 
 // Drucker-Prager return map for crushed points (Wolper_Drucker_Prager
 // :48-94).  Updates Fe and Jp_inv in place.
-__device__ __forceinline__ void dp_return(const KParams &k, const Trial &r, double &F00,
+static __device__ __forceinline__ void dp_return(const KParams &k, const Trial &r, double &F00,
                                           double &F01, double &F10, double &F11,
                                           double &Jp_inv) {
     if (r.p < -k.dpThr || Jp_inv < 1) {
@@ -206,7 +208,7 @@ __device__ __forceinline__ void dp_return(const KParams &k, const Trial &r, doub
 // ----------------------------------------------------------------------------
 // Quadratic B-spline stencil along one axis (gpu_implementation5.cu:185-193)
 // Returns false for a position whose base index cannot be represented.
-__device__ __forceinline__ bool stencil(double x, double h_inv, int &base, double &fx,
+static __device__ __forceinline__ bool stencil(double x, double h_inv, int &base, double &fx,
                                         double &w0, double &w1, double &w2) {
     double bx = x * h_inv - 0.5;
     if (!(bx > -1.0 && bx < 2147483000.0)) return false;
@@ -221,14 +223,16 @@ __device__ __forceinline__ bool stencil(double x, double h_inv, int &base, doubl
     return true;
 }
 
-__device__ __forceinline__ double sel3(int i, double a, double b, double c) {
+static __device__ __forceinline__ double sel3(int i, double a, double b, double c) {
     return i == 0 ? a : (i == 1 ? b : c);
 }
 
 // Cell key of a position (SnapshotManager::SortPoints key,
 // snapshotmanager.cpp:300-301, on the kernel's base index)
-__device__ __forceinline__ int cell_key(double x, double y, double hinv, int gridX) {
+static __device__ __forceinline__ int cell_key(double x, double y, double hinv, int gridX) {
     return (int)(x * hinv - 0.5) + ((int)(y * hinv - 0.5)) * gridX;
 }
+
+static __device__ __forceinline__ void flag_error(const KParams &k) { k.err[0] = 1; }
 
 }  // namespace icempm

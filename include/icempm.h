@@ -160,6 +160,16 @@ int icempm_sort_raw_points(icempm_ctx *ctx, float *xy, int16_t *grains, int n, d
 int icempm_set_mode(icempm_ctx *ctx, int mode);
 int icempm_get_mode(const icempm_ctx *ctx);
 
+/* Tuning knobs (no counterpart).  ICEMPM_TUNE_PASS_KIND selects the kernel
+ * of the particle pass: 0 workgroup-wide shared-memory windows for gather and
+ * scatter, 1 wavefront windows (direct gather, per-wavefront scatter), 2 as 1
+ * with the per-lane stencil rotation (default).  Results differ only in the
+ * order of the FAST-mode P2G sums. */
+enum { ICEMPM_TUNE_PASS_KIND = 0 };
+int icempm_set_tuning(icempm_ctx *ctx, int knob, int value);
+/* current value, or -1 for an unknown knob */
+int icempm_get_tuning(const icempm_ctx *ctx, int knob);
+
 /* Device-side timing of the dominant kernel (the fused G2P+P2G pass, or
  * G2P in ORDERED mode) over icempm_run_substeps calls: when enabled, each
  * launch is bracketed by events on the context's stream and the durations

@@ -148,17 +148,27 @@ def test_fast_p2g_single_call(gpu_available):
     say("fast p2g")
     case = INDENT_60K
     m, p, pts = load_pair(case, MODE_FAST)
-    m.run(30)
     grid = oracle.new_grid(p)
     oracle.run(p, pts, grid, np.zeros(200), 30)
+    # same state on both sides: upload the oracle's points
+    m.host_points()[:] = pts
     H = _lib.hip()
+    _lib.check_ctx(H.icempm_transfer_points_to_device(m.ctx), m.ctx)
     _lib.check_ctx(H.icempm_reset_grid(m.ctx), m.ctx)
     _lib.check_ctx(H.icempm_p2g(m.ctx), m.ctx)
     g_ref = oracle.new_grid(p)
     oracle.p2g(p, pts, g_ref)
     g = m.grid()
-    scale = np.abs(g_ref).max(axis=1, keepdims=True)
-    assert (np.abs(g - g_ref) <= 1e-13 * scale).all()
+    # FAST mode sums in another order and contracts a*b+c into FMAs; the
+    # stress term kappa*(J*J-1) with J ~ 1-1e-5 loses ~5 digits without FMA,
+    # so per-node differences reach ~1e-11 of the row scale.  A dropped or
+    # doubled contribution would be ~1e-3.
+    scale = np.abs(g_ref).max(axis=1)
+    err = np.abs(g - g_ref).max(axis=1) / scale
+    say("fast p2g: max error / row scale", err)
+    assert (err <= 1e-9).all()
+    assert np.array_equal(g[0] != 0, g_ref[0] != 0)
+    assert g[0].sum() == pytest.approx(g_ref[0].sum(), rel=1e-13)
     m.close()
 
 
