@@ -151,6 +151,9 @@ def main() -> int:
     log(f"bench: {ms_per_step * 1e3:.1f} us/substep, fused kernel {k_avg_s * 1e6:.1f} us "
         f"({klaunch} timed launches), {achieved:.0f} GB/s")
 
+    copy_gbs = m.copy_bandwidth(1 << 30, 20)
+    log(f"bench: device copy kernel {copy_gbs:.0f} GB/s (read + write)")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -190,6 +193,7 @@ def main() -> int:
             "bytes_per_launch": k_bytes,
             "avg_launch_us": k_avg_s * 1e6,
             "timed_launches": klaunch,
+            "copy_kernel_gbs": copy_gbs,
         },
         "substep_effective_gbs": sub_bytes * args.steps / el / 1e9,
         "cpu_baseline": cpu,

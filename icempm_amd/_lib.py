@@ -118,6 +118,7 @@ HIP_EXPORTS = [
     "icempm_sort_raw_points", "icempm_set_mode", "icempm_get_mode", "icempm_kernel_timing",
     "icempm_kernel_timing_read", "icempm_read_grid", "icempm_read_points",
     "icempm_write_grid", "icempm_read_bins", "icempm_set_tuning", "icempm_get_tuning",
+    "icempm_measure_copy",
 ]
 
 HOST_EXPORTS = [
@@ -177,6 +178,7 @@ def hip():
     _sig(L, "icempm_read_bins", c_int, P, POINTER(c_double))
     _sig(L, "icempm_set_tuning", c_int, P, c_int, c_int)
     _sig(L, "icempm_get_tuning", c_int, P, c_int)
+    _sig(L, "icempm_measure_copy", c_int, P, c_size_t, c_int, POINTER(c_double))
     _hip = L
     return L
 

@@ -5,6 +5,7 @@
 // device arrays, the pinned host buffers and the device error flag.  All
 // device work is queued on the context's stream.
 #include <atomic>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -35,6 +36,7 @@ struct icempm_ctx {
     double *bins = nullptr;
     int nBinsAlloc = 0;   // doubles
     int *dErr = nullptr;
+    double *grainTab = nullptr;   // kGrainEntries x kGrainFields
     double *hist = nullptr;
     size_t histCap = 0;   // doubles
     int lastRunSubsteps = 0;
@@ -106,6 +108,7 @@ KParams make_kparams(const icempm_ctx *c) {
     const icy::SimParams &P = c->prms;
     KParams k{};
     k.pts = c->pts;
+    k.grainTab = c->grainTab;
     k.pp = c->pp;
     k.gp = c->gp;
     k.bins = c->bins;
@@ -134,6 +137,36 @@ KParams make_kparams(const icempm_ctx *c) {
     k.tanPhi = P.DP_tan_phi;
     k.dpThr = P.DP_threshold_p;
     return k;
+}
+
+// Per-grain constants of the NACC test for every short grain id, evaluated in
+// the order of GetParametersForGrain / CheckIfPointIsInsideFailureSurface
+// (gpu_implementation5.cu:97-141), so that the table lookup returns exactly
+// what the per-particle formula would (this file is built without FMA
+// contraction; sqrt and division are IEEE on host and device alike).
+void grain_table(const icy::SimParams &P, std::vector<double> &t) {
+    t.assign((size_t)kGrainEntries * kGrainFields, 0.0);
+    for (int gi = -32768; gi < 32768; ++gi) {
+        const short grain = (short)gi;
+        const double var1 = 1.0 + P.GrainVariability * 0.05 * (-10 + grain % 21);
+        const double var2 = 1.0 + P.GrainVariability * 0.033 * (-15 + (grain + 3) % 30);
+        const double var3 = 1.0 + P.GrainVariability * 0.1 * (-10 + (grain + 4) % 11);
+        const double pmax = P.IceCompressiveStrength * var1;
+        const double pmin = -P.IceTensileStrength * var2;
+        const double qmax = P.IceShearStrength * var3;
+        const double beta = -pmin / pmax;
+        const double a = pmax * (1 + beta);
+        const double mSq = (4 * qmax * qmax * (1 + 2 * beta)) / (a * a);
+        const double pmin2 = -3e6;
+        const double q0 = 2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin);
+        const double slope = -q0 / pmin2;
+        double *row = t.data() + (size_t)(gi + 32768) * kGrainFields;
+        row[kGPmax] = pmax;
+        row[kGMsq] = mSq;
+        row[kGBetaPmax] = beta * pmax;
+        row[kGOnePlus2Beta] = 1. + 2. * beta;
+        row[kGSlope] = slope;
+    }
 }
 
 // Launch preconditions shared by every substep operation.
@@ -382,6 +415,7 @@ void icempm_destroy(icempm_ctx *c) {
     if (c->evStart) (void)hipEventDestroy(c->evStart);
     if (c->evStop) (void)hipEventDestroy(c->evStop);
     dfree(c->dErr);
+    dfree(c->grainTab);
     hfree(c->hostErr);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -420,6 +454,16 @@ int icempm_update_constants(icempm_ctx *c, const void *simparams, size_t nbytes)
     if (nbytes != sizeof(icy::SimParams))
         return fail(c, ICEMPM_E_ARG, "cuda_update_constants: SimParams size mismatch");
     std::memcpy(&c->prms, simparams, sizeof(icy::SimParams));
+    if (!c->grainTab)
+        HIPCHK(c, hipMalloc(&c->grainTab, (size_t)kGrainEntries * kGrainFields * sizeof(double)),
+               "cuda_update_constants: gprms");
+    {
+        std::vector<double> t;
+        grain_table(c->prms, t);
+        HIPCHK(c, hipMemcpyAsync(c->grainTab, t.data(), t.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream), "cuda_update_constants: gprms");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "cuda_update_constants: gprms");
+    }
     c->haveParams = true;
     c->chunksDirty = true;
     // gpu_error_indicator <- error_code (gpu_implementation5.cu:474)
@@ -752,6 +796,34 @@ int icempm_kernel_timing_read(icempm_ctx *c, double *total_ms, int *launches) {
     if (!c || !total_ms || !launches) return ICEMPM_E_ARG;
     *total_ms = c->timedMs;
     *launches = c->timedLaunches;
+    return ICEMPM_OK;
+}
+
+int icempm_measure_copy(icempm_ctx *c, size_t nbytes, int reps, double *gbps) {
+    if (!c || !gbps || reps <= 0 || nbytes < 1024) return ICEMPM_E_ARG;
+    const size_t n = nbytes / sizeof(double) / 2 * 2;
+    double *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&b, n * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_fill_f64(a, n, 1.0, c->stream);
+    if (e == hipSuccess) e = launch_copy(a, b, n, c->stream);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(a, b, n, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipStreamSynchronize(c->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
+    if (e != hipSuccess) return hipfail(c, e, "measure_copy");
+    // bytes read + bytes written per copy
+    *gbps = 2.0 * (double)n * sizeof(double) * reps / (ms * 1e-3) / 1e9;
     return ICEMPM_OK;
 }
 

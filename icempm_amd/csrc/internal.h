@@ -69,6 +69,7 @@ hipError_t ordered_fold_forces(const KParams &k, OrderedScratch &o, double *hist
                                hipStream_t s);
 
 hipError_t launch_hello(int *flag, hipStream_t s);
+hipError_t launch_copy(const double *src, double *dst, size_t n_doubles, hipStream_t s);
 
 // ---- sort.hip -----------------------------------------------------------------
 // Exclusive scan of n ints (in and out may alias).  tmp must hold

@@ -25,6 +25,7 @@ constexpr double kSqrt2 = 1.4142135623730950;    // gpu_implementation5.cu:22 (c
 // and is read into SGPRs).
 struct KParams {
     double *pts;        // 16 rows x pp doubles
+    const double *grainTab;   // kGrainEntries x kGrainFields per-grain strengths
     size_t pp;          // point pitch (doubles)
     size_t gp;          // grid pitch (doubles)
     double *bins;       // 2 * nbins doubles
@@ -144,6 +145,29 @@ static __device__ __forceinline__ Trial compute_pq(double F00, double F01, doubl
     r.sh1 = c * (r.ss1 - half_sum);
     r.q = kSqrt2 * sqrt(r.sh0 * r.sh0 + r.sh1 * r.sh1);
     return r;
+}
+
+// Per-grain constants of the NACC test, one row per short grain id
+// (index grain + 32768), computed on the host in the order of
+// GetParametersForGrain (gpu_implementation5.cu:97-111) and
+// CheckIfPointIsInsideFailureSurface (:114-141): pmax, mSq, beta*pmax,
+// (1 + 2 beta), and the slope of the tension line -q0/pmin2.
+constexpr int kGrainEntries = 65536;
+constexpr int kGrainFields = 5;
+enum { kGPmax = 0, kGMsq = 1, kGBetaPmax = 2, kGOnePlus2Beta = 3, kGSlope = 4 };
+
+// the same test as fails(), reading the per-grain constants from the table
+static __device__ __forceinline__ bool fails_tab(const KParams &k, short grain, double p, double q) {
+    const double *g = k.grainTab + ((int)grain + 32768) * kGrainFields;
+    const double pmin2 = -3e6;
+    if (p < 0) {
+        if (p < pmin2) return true;
+        const double q_limit = g[kGSlope] * (p - pmin2);
+        return q > q_limit;
+    }
+    const double pmax = g[kGPmax];
+    const double y = g[kGOnePlus2Beta] * q * q + g[kGMsq] * (p + g[kGBetaPmax]) * (p - pmax);
+    return y > 0;
 }
 
 // NACC failure test for intact points with per-grain strengths

@@ -10,4 +10,5 @@
 #include "internal.h"
 
 #define ICEMPM_PASS_NS fast
+#define ICEMPM_FAST_ALGEBRA 1
 #include "particles.inc"

@@ -177,6 +177,10 @@ int icempm_get_tuning(const icempm_ctx *ctx, int knob);
 int icempm_kernel_timing(icempm_ctx *ctx, int enable);
 int icempm_kernel_timing_read(icempm_ctx *ctx, double *total_ms, int *launches);
 
+/* Device copy bandwidth (no counterpart): reps launches of a plain streaming
+ * copy kernel (k_copy) over nbytes; *gbps = (read + written bytes) / time. */
+int icempm_measure_copy(icempm_ctx *ctx, size_t nbytes, int reps, double *gbps);
+
 /* Raw device buffers for tests: copies the device grid (3*nGridPitch
  * doubles) or the point SoA (16*nPtsPitch doubles) to host memory. */
 int icempm_read_grid(icempm_ctx *ctx, double *out);
