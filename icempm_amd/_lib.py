@@ -23,6 +23,8 @@ ICEMPM_E_NODEV = 4
 MODE_FAST = 0
 MODE_ORDERED = 1
 TUNE_PASS_KIND = 0
+BATCH_F64 = 0
+BATCH_F32 = 1
 
 
 class IcempmError(RuntimeError):
@@ -118,7 +120,14 @@ HIP_EXPORTS = [
     "icempm_sort_raw_points", "icempm_set_mode", "icempm_get_mode", "icempm_kernel_timing",
     "icempm_kernel_timing_read", "icempm_read_grid", "icempm_read_points",
     "icempm_write_grid", "icempm_read_bins", "icempm_set_tuning", "icempm_get_tuning",
-    "icempm_measure_copy",
+    "icempm_measure_copy", "icempm_batch_create", "icempm_batch_destroy",
+    "icempm_batch_last_error", "icempm_batch_add_case", "icempm_batch_finalize",
+    "icempm_batch_run", "icempm_batch_last_ms", "icempm_batch_num_points",
+    "icempm_batch_force_history", "icempm_batch_read_points", "icempm_batch_error_flag",
+    "icempm_mlp_num_params", "icempm_mlp_create", "icempm_mlp_destroy", "icempm_mlp_last_error",
+    "icempm_mlp_get_params", "icempm_mlp_device_params", "icempm_mlp_step", "icempm_mlp_infer",
+    "icempm_mfma_probe", "icempm_device_alloc", "icempm_device_free", "icempm_copy_to_device",
+    "icempm_copy_to_host",
 ]
 
 HOST_EXPORTS = [
@@ -179,6 +188,31 @@ def hip():
     _sig(L, "icempm_set_tuning", c_int, P, c_int, c_int)
     _sig(L, "icempm_get_tuning", c_int, P, c_int)
     _sig(L, "icempm_measure_copy", c_int, P, c_size_t, c_int, POINTER(c_double))
+    _sig(L, "icempm_batch_create", c_int, POINTER(c_void_p), c_int)
+    _sig(L, "icempm_batch_destroy", None, P)
+    _sig(L, "icempm_batch_last_error", c_char_p, P)
+    _sig(L, "icempm_batch_add_case", c_int, P, c_void_p, c_size_t, POINTER(c_double), c_size_t)
+    _sig(L, "icempm_batch_finalize", c_int, P)
+    _sig(L, "icempm_batch_run", c_int, P, c_int, c_int)
+    _sig(L, "icempm_batch_last_ms", c_int, P, POINTER(c_float))
+    _sig(L, "icempm_batch_num_points", c_int, P)
+    _sig(L, "icempm_batch_force_history", c_int, P, c_int, POINTER(c_double), c_int)
+    _sig(L, "icempm_batch_read_points", c_int, P, c_int, POINTER(c_double), c_size_t)
+    _sig(L, "icempm_batch_error_flag", c_int, P, POINTER(c_int))
+    fp = POINTER(c_float)
+    _sig(L, "icempm_mlp_num_params", c_int)
+    _sig(L, "icempm_mlp_create", c_int, POINTER(c_void_p), fp)
+    _sig(L, "icempm_mlp_destroy", None, P)
+    _sig(L, "icempm_mlp_last_error", c_char_p, P)
+    _sig(L, "icempm_mlp_get_params", c_int, P, fp)
+    _sig(L, "icempm_mlp_device_params", c_void_p, P)
+    _sig(L, "icempm_mlp_step", c_int, P, c_void_p, c_void_p, c_int, c_int, c_float, fp, fp)
+    _sig(L, "icempm_mlp_infer", c_int, P, c_void_p, c_int, c_void_p)
+    _sig(L, "icempm_mfma_probe", c_int, fp, fp, fp)
+    _sig(L, "icempm_device_alloc", c_int, c_size_t, POINTER(c_void_p))
+    _sig(L, "icempm_device_free", None, c_void_p)
+    _sig(L, "icempm_copy_to_device", c_int, c_void_p, c_void_p, c_size_t)
+    _sig(L, "icempm_copy_to_host", c_int, c_void_p, c_void_p, c_size_t)
     _hip = L
     return L
 

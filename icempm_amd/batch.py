@@ -1,0 +1,125 @@
+"""Batches of independent indentation cases on one GPU (C ABI
+``icempm_batch_*``): many cases of uneven size advanced together, one grid
+update and one particle-pass launch per substep for all of them, the substep
+loop captured in a HIP graph.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from ctypes import POINTER, byref, c_double, c_float, c_int, c_void_p
+
+import numpy as np
+
+from . import _lib
+from ._lib import BATCH_F32, BATCH_F64, IcempmError, SimParams
+from .model import MODE_FAST, IndentationCase, Model, nacc_strengths
+
+
+def batch_cases(n_cases: int, n_min: int, n_max: int, seed: int, h: float = 0.012,
+                dt: float = 2e-6, e_range=(5e9, 10e9), tensile_range=(0.5e6, 2e6),
+                speed_range=(0.1, 1.0)) -> list[IndentationCase]:
+    """Seeded cases: point counts log-uniform in [n_min, n_max] (4 points per
+    cell, block aspect 8:3 like the 2.4 m x 0.9 m block), Young's modulus,
+    tensile strength (NACC M = 1.2, beta = 0.8 kept) and disc speed uniform in
+    their ranges; grid margins as in the 60k case (56 columns, 53 rows)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_cases):
+        n = math.exp(rng.uniform(math.log(n_min), math.log(n_max)))
+        cells = n / 4
+        ncy = max(8, int(round(math.sqrt(cells * 3 / 8))))
+        ncx = max(16, int(round(cells / ncy)))
+        E = rng.uniform(*e_range)
+        T = rng.uniform(*tensile_range)
+        V = rng.uniform(*speed_range)
+        C, T2, S = nacc_strengths(1.2, 0.8, T)
+        out.append(IndentationCase(
+            name=f"batch{seed}-{i}", ncx=ncx, ncy=ncy, per_side=2, h=h, grid_x=ncx + 56,
+            grid_y=ncy + 53, dt=dt, steps=2000, youngs=E, compressive=C, tensile=T2, shear=S,
+            ind_diameter=0.3, ind_velocity=V, ind_depth=0.1, seed=0x1CE5EED + i))
+    return out
+
+
+class Batch:
+    def __init__(self, precision: int = BATCH_F64):
+        self._H = _lib.hip()
+        b = c_void_p()
+        rc = self._H.icempm_batch_create(byref(b), precision)
+        if rc != 0:
+            raise IcempmError(rc, "icempm_batch_create failed")
+        self._b = b
+        self.precision = precision
+        self.cases: list[IndentationCase] = []
+        self.initial: list[tuple[SimParams, np.ndarray]] = []
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            msg = self._H.icempm_batch_last_error(self._b) or b""
+            raise IcempmError(rc, msg.decode(errors="replace"))
+
+    def close(self) -> None:
+        if getattr(self, "_b", None):
+            self._H.icempm_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, model: Model, case: IndentationCase) -> int:
+        """Ingest the case through the model (LoadRawPoints) and add its state."""
+        model.load_case(case)
+        soa = np.ascontiguousarray(model.points())
+        p = model.prms.copy()
+        idx = self._H.icempm_batch_add_case(self._b, byref(p), ctypes.sizeof(SimParams),
+                                            soa.ctypes.data_as(POINTER(c_double)), soa.shape[1])
+        if idx < 0:
+            self._check(-idx)
+        self.cases.append(case)
+        self.initial.append((p, soa))
+        return idx
+
+    def finalize(self) -> None:
+        self._check(self._H.icempm_batch_finalize(self._b))
+
+    def run(self, n: int, use_graph: bool = True) -> float:
+        """n substeps of every case; returns the device time in ms."""
+        self._check(self._H.icempm_batch_run(self._b, n, 1 if use_graph else 0))
+        ms = c_float(0)
+        self._check(self._H.icempm_batch_last_ms(self._b, byref(ms)))
+        return ms.value
+
+    @property
+    def num_points(self) -> int:
+        return int(self._H.icempm_batch_num_points(self._b))
+
+    def force_history(self, c: int, n: int) -> np.ndarray:
+        out = np.zeros((n, 2))
+        self._check(self._H.icempm_batch_force_history(self._b, c, out.ctypes.data_as(POINTER(c_double)), n))
+        return out
+
+    def points(self, c: int) -> np.ndarray:
+        pitch = self.initial[c][1].shape[1]
+        out = np.zeros((16, pitch))
+        self._check(self._H.icempm_batch_read_points(self._b, c, out.ctypes.data_as(POINTER(c_double)), pitch))
+        return out
+
+    def error_flag(self) -> int:
+        f = c_int(0)
+        self._check(self._H.icempm_batch_error_flag(self._b, byref(f)))
+        return f.value
+
+
+def build_batch(cases: list[IndentationCase], precision: int = BATCH_F64) -> Batch:
+    b = Batch(precision)
+    m = Model(MODE_FAST)
+    try:
+        for c in cases:
+            b.add(m, c)
+    finally:
+        m.close()
+    b.finalize()
+    return b

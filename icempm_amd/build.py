@@ -21,7 +21,8 @@ CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 OBJ = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["api.hip", "kernels.hip", "particles_exact.hip", "particles_fast.hip", "sort.hip"]
+HIP_SOURCES = ["api.hip", "kernels.hip", "particles_exact.hip", "particles_fast.hip", "sort.hip",
+               "batch.hip", "mlp.hip"]
 HOST_SOURCES = ["parameters_sim.cpp", "gpu_implementation5.cpp", "model.cpp", "snapshotmanager.cpp",
                 "scene.cpp", "host_capi.cpp"]
 

@@ -1,0 +1,474 @@
+// mlp.hip -- the learned return map: a 5 -> 64 -> 64 -> 3 MLP with tanh
+// hidden layers, BF16 weights and activations on the MFMA units
+// (v_mfma_f32_16x16x16_bf16, FP32 accumulation), trained on the GPU by the
+// kernels below (forward, backward, Adam) and evaluated by k_mlp_infer and by
+// the batch particle pass.
+//
+// Parameters (one flat FP32 vector, also kept as BF16 for the MFMA operands):
+//   W1[64][16]  (inputs padded 5 -> 16 with zero columns)   offset 0
+//   b1[64]                                                  offset 1024
+//   W2[64][64]                                              offset 1088
+//   b2[64]                                                  offset 5184
+//   W3[16][64]  (outputs padded 3 -> 16 with zero rows)     offset 5248
+//   b3[16]                                                  offset 6272
+// W[out][in]: y = W x + b.  Inputs and outputs are normalised by the caller.
+//
+// MFMA 16x16x16 lane layout (wavefront of 64): lane l holds A[l%16][4g..4g+3]
+// and B[4g..4g+3][l%16] with g = l/16, and receives D[4g+i][l%16], i < 4.
+// tests/test_gpu_mlp.py checks this layout against a CPU product.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/icempm.h"
+#include "internal.h"
+
+namespace icempm {
+namespace mlp {
+
+constexpr int kIn = 16, kH = 64, kOut = 16;
+constexpr int oW1 = 0, oB1 = 1024, oW2 = 1088, oB2 = 5184, oW3 = 5248, oB3 = 6272, kParams = 6288;
+constexpr int kRealIn = 5, kRealOut = 3;
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bits, round to nearest even (NaN kept quiet)
+static __device__ __host__ __forceinline__ unsigned short to_bf16(float f) {
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (unsigned short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+// One 16x16 output tile of A(16 x K) * B(K x 16) accumulated into acc.
+// a(m, k) and b(k, n) return the operands as floats (rounded to BF16 here).
+template <class FA, class FB>
+static __device__ __forceinline__ f4 tile(FA a, FB b, int K, f4 acc) {
+    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        s4 av, bv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            av[j] = (short)to_bf16(a(r, k0 + 4 * g + j));
+            bv[j] = (short)to_bf16(b(k0 + 4 * g + j, r));
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Lane layout probe: D = A * B for one 16x16x16 tile, A and B row-major.
+__global__ void k_mfma_probe(const float *A, const float *B, float *D) {
+    const int l = threadIdx.x, g = l >> 4, c = l & 15;
+    f4 acc = {0, 0, 0, 0};
+    acc = tile([&](int m, int k) { return A[m * 16 + k]; }, [&](int k, int n) { return B[k * 16 + n]; },
+               16, acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + c] = acc[i];
+}
+
+// Per-wavefront scratch in shared memory (floats).
+struct WaveScratch {
+    float x[16][kIn];
+    float h1[16][kH];
+    float h2[16][kH];
+    float y[16][kOut];
+    float d2[16][kH];   // dL/d(pre-activation 2)
+    float d1[16][kH];   // dL/d(pre-activation 1)
+};
+
+// store a D tile (lane l: rows 4g+i, column c) into a row-major array
+template <int LD>
+static __device__ __forceinline__ void put(float (*dst)[LD], int n0, f4 v) {
+    const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dst[4 * g + i][n0 + c] = v[i];
+}
+
+// forward of one 16-row tile held in s.x; parameters P (float, shared or global)
+template <class PW>
+static __device__ __forceinline__ void forward16(WaveScratch &s, const PW &P) {
+    const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+    for (int nt = 0; nt < kH / 16; ++nt) {
+        f4 acc = {0, 0, 0, 0};
+        acc = tile([&](int m, int k) { return s.x[m][k]; },
+                   [&](int k, int n) { return P[oW1 + (nt * 16 + n) * kIn + k]; }, kIn, acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = tanhf(acc[i] + P[oB1 + nt * 16 + c]);
+        put<kH>(s.h1, nt * 16, acc);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int nt = 0; nt < kH / 16; ++nt) {
+        f4 acc = {0, 0, 0, 0};
+        acc = tile([&](int m, int k) { return s.h1[m][k]; },
+                   [&](int k, int n) { return P[oW2 + (nt * 16 + n) * kH + k]; }, kH, acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = tanhf(acc[i] + P[oB2 + nt * 16 + c]);
+        put<kH>(s.h2, nt * 16, acc);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    {
+        f4 acc = {0, 0, 0, 0};
+        acc = tile([&](int m, int k) { return s.h2[m][k]; },
+                   [&](int k, int n) { return P[oW3 + n * kH + k]; }, kH, acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] += P[oB3 + c];
+        put<kOut>(s.y, 0, acc);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    (void)g;
+}
+
+struct GlobalParams {
+    const float *p;
+    __device__ float operator[](int i) const { return p[i]; }
+};
+
+// Inference: rows of 16 normalised inputs (only the first 5 used) -> 3
+// normalised outputs (stride 3).
+__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ params,
+                                                   const float *__restrict__ X, int n,
+                                                   float *__restrict__ Y) {
+    __shared__ WaveScratch ws[4];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    WaveScratch &s = ws[w];
+    const GlobalParams P{params};
+    const int tiles = (n + 15) / 16;
+    for (int t = blockIdx.x * 4 + w; t < tiles; t += gridDim.x * 4) {
+        for (int e = l; e < 16 * kIn; e += 64) {
+            const int m = e / kIn, k = e % kIn, row = t * 16 + m;
+            s.x[m][k] = (row < n && k < kRealIn) ? X[(size_t)row * kRealIn + k] : 0.f;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        forward16(s, P);
+        for (int e = l; e < 16 * kRealOut; e += 64) {
+            const int m = e / kRealOut, k = e % kRealOut, row = t * 16 + m;
+            if (row < n) Y[(size_t)row * kRealOut + k] = s.y[m][k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Gradient of L = 1/(2B) sum_rows sum_{k<3} (y_k - t_k)^2 over rows
+// [r0, r0 + B): every wavefront accumulates its tiles' gradient in registers
+// and writes one partial vector (kParams floats) to part[wave].  Also adds
+// the tiles' loss to *loss.
+__global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ params,
+                                                  const float *__restrict__ X,
+                                                  const float *__restrict__ T, int r0, int B,
+                                                  float *__restrict__ part, float *__restrict__ loss) {
+    __shared__ WaveScratch ws[4];
+    __shared__ float P[kParams];
+    for (int e = threadIdx.x; e < kParams; e += 256) P[e] = params[e];
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+    WaveScratch &s = ws[w];
+    const float invB = 1.f / (float)B;
+    f4 gW1[4], gW2[16], gW3[4];
+    float gb1[4], gb2[4], gb3 = 0;
+    for (int i = 0; i < 4; ++i) {
+        gW1[i] = f4{0, 0, 0, 0};
+        gW3[i] = f4{0, 0, 0, 0};
+        gb1[i] = 0;
+        gb2[i] = 0;
+    }
+    for (int i = 0; i < 16; ++i) gW2[i] = f4{0, 0, 0, 0};
+    float lsum = 0;
+    const int tiles = (B + 15) / 16;
+    const int wave = blockIdx.x * 4 + w, nwave = gridDim.x * 4;
+    for (int t = wave; t < tiles; t += nwave) {
+        for (int e = l; e < 16 * kIn; e += 64) {
+            const int m = e / kIn, k = e % kIn, row = t * 16 + m;
+            s.x[m][k] = (row < B && k < kRealIn) ? X[(size_t)(r0 + row) * kRealIn + k] : 0.f;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        forward16(s, P);
+        // dY (kept in s.y), zero for padded outputs and rows
+        for (int e = l; e < 16 * kOut; e += 64) {
+            const int m = e / kOut, k = e % kOut, row = t * 16 + m;
+            float d = 0;
+            if (row < B && k < kRealOut) {
+                d = s.y[m][k] - T[(size_t)(r0 + row) * kRealOut + k];
+                lsum += 0.5f * d * d;
+                d *= invB;
+            }
+            s.y[m][k] = d;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // layer 3: gW3[o][h] += sum_m dY[m][o] h2[m][h]; gb3[o] += sum_m dY[m][o]
+        for (int ht = 0; ht < 4; ++ht)
+            gW3[ht] = tile([&](int o, int m) { return s.y[m][o]; },
+                           [&](int m, int h) { return s.h2[m][ht * 16 + h]; }, 16, gW3[ht]);
+        // d2 = (dY W3) * (1 - h2^2)
+        for (int ht = 0; ht < 4; ++ht) {
+            f4 acc = {0, 0, 0, 0};
+            acc = tile([&](int m, int o) { return s.y[m][o]; },
+                       [&](int o, int h) { return P[oW3 + o * kH + ht * 16 + h]; }, 16, acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float hv = s.h2[4 * g + i][ht * 16 + c];
+                acc[i] *= (1.f - hv * hv);
+            }
+            put<kH>(s.d2, ht * 16, acc);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // layer 2: gW2[o][h] += sum_m d2[m][o] h1[m][h]
+        for (int ot = 0; ot < 4; ++ot)
+            for (int ht = 0; ht < 4; ++ht)
+                gW2[ot * 4 + ht] = tile([&](int o, int m) { return s.d2[m][ot * 16 + o]; },
+                                        [&](int m, int h) { return s.h1[m][ht * 16 + h]; }, 16,
+                                        gW2[ot * 4 + ht]);
+        // d1 = (d2 W2) * (1 - h1^2)
+        for (int ht = 0; ht < 4; ++ht) {
+            f4 acc = {0, 0, 0, 0};
+            acc = tile([&](int m, int o) { return s.d2[m][o]; },
+                       [&](int o, int h) { return P[oW2 + o * kH + ht * 16 + h]; }, kH, acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float hv = s.h1[4 * g + i][ht * 16 + c];
+                acc[i] *= (1.f - hv * hv);
+            }
+            put<kH>(s.d1, ht * 16, acc);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // layer 1: gW1[o][k] += sum_m d1[m][o] x[m][k]
+        for (int ot = 0; ot < 4; ++ot)
+            gW1[ot] = tile([&](int o, int m) { return s.d1[m][ot * 16 + o]; },
+                           [&](int m, int k) { return s.x[m][k]; }, 16, gW1[ot]);
+        // biases: column sums (lanes with g == 0 keep them)
+        for (int i = 0; i < 4; ++i) {
+            float a = 0, b = 0;
+            for (int m = g; m < 16; m += 4) {
+                a += s.d1[m][i * 16 + c];
+                b += s.d2[m][i * 16 + c];
+            }
+            a += __shfl_xor(a, 16, 64);
+            a += __shfl_xor(a, 32, 64);
+            b += __shfl_xor(b, 16, 64);
+            b += __shfl_xor(b, 32, 64);
+            gb1[i] += a;
+            gb2[i] += b;
+        }
+        {
+            float a = 0;
+            for (int m = g; m < 16; m += 4) a += s.y[m][c];
+            a += __shfl_xor(a, 16, 64);
+            a += __shfl_xor(a, 32, 64);
+            gb3 += a;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    // write this wavefront's partial gradient
+    float *out = part + (size_t)wave * kParams;
+    for (int ot = 0; ot < 4; ++ot)
+        for (int i = 0; i < 4; ++i) out[oW1 + (ot * 16 + 4 * g + i) * kIn + c] = gW1[ot][i];
+    for (int ot = 0; ot < 4; ++ot)
+        for (int ht = 0; ht < 4; ++ht)
+            for (int i = 0; i < 4; ++i)
+                out[oW2 + (ot * 16 + 4 * g + i) * kH + ht * 16 + c] = gW2[ot * 4 + ht][i];
+    for (int ht = 0; ht < 4; ++ht)
+        for (int i = 0; i < 4; ++i) out[oW3 + (4 * g + i) * kH + ht * 16 + c] = gW3[ht][i];
+    if (g == 0) {
+        for (int i = 0; i < 4; ++i) {
+            out[oB1 + i * 16 + c] = gb1[i];
+            out[oB2 + i * 16 + c] = gb2[i];
+        }
+        out[oB3 + c] = gb3;
+    }
+    // the loss of this wavefront's rows
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
+    if (l == 0) atomicAdd(loss, lsum);
+}
+
+// grad = sum over the partial vectors; then one Adam step on the FP32
+// master parameters.  Parameters that are padding stay zero.
+__global__ void k_mlp_adam(float *__restrict__ params, float *__restrict__ mom1,
+                           float *__restrict__ mom2, const float *__restrict__ part, int nparts,
+                           float *__restrict__ grad, float lr, float b1, float b2, float eps,
+                           float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kParams) return;
+    float gsum = 0;
+    for (int p = 0; p < nparts; ++p) gsum += part[(size_t)p * kParams + i];
+    grad[i] = gsum;
+    // padding: input columns >= 5 of W1, output rows >= 3 of W3 and b3
+    bool pad = false;
+    if (i < oB1) pad = (i % kIn) >= kRealIn;
+    else if (i >= oW3 && i < oB3) pad = ((i - oW3) / kH) >= kRealOut;
+    else if (i >= oB3) pad = (i - oB3) >= kRealOut;
+    if (pad) {
+        params[i] = 0;
+        return;
+    }
+    if (lr == 0) return;
+    const float m = b1 * mom1[i] + (1 - b1) * gsum;
+    const float v = b2 * mom2[i] + (1 - b2) * gsum * gsum;
+    mom1[i] = m;
+    mom2[i] = v;
+    params[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+}
+
+}  // namespace mlp
+}  // namespace icempm
+
+using namespace icempm::mlp;
+
+struct icempm_mlp {
+    hipStream_t stream = nullptr;
+    float *params = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr, *part = nullptr;
+    float *loss = nullptr;
+    int nparts = 0;
+    int t = 0;   // Adam step
+    std::string lastError;
+};
+
+namespace {
+constexpr int kGradBlocks = 128;
+
+int mfail(icempm_mlp *m, hipError_t e, const char *tag) {
+    if (m) m->lastError = std::string(tag) + ": " + hipGetErrorString(e);
+    return ICEMPM_E_HIP;
+}
+#define MCHK(m, call, tag)                               \
+    do {                                                 \
+        hipError_t _e = (call);                          \
+        if (_e != hipSuccess) return mfail(m, _e, tag);  \
+    } while (0)
+}  // namespace
+
+extern "C" {
+
+int icempm_mlp_num_params(void) { return kParams; }
+
+int icempm_mlp_create(icempm_mlp **out, const float *init) {
+    if (!out || !init) return ICEMPM_E_ARG;
+    icempm_mlp *m = new icempm_mlp();
+    m->nparts = kGradBlocks * 4;
+    hipError_t e = hipSetDevice(0);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&m->params, kParams * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->m1, kParams * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->m2, kParams * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->grad, kParams * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->part, (size_t)m->nparts * kParams * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->loss, sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(m->params, init, kParams * sizeof(float), hipMemcpyHostToDevice);
+    std::vector<float> z(kParams, 0.f);
+    if (e == hipSuccess) e = hipMemcpy(m->m1, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->m2, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        icempm_mlp_destroy(m);
+        return ICEMPM_E_HIP;
+    }
+    *out = m;
+    return ICEMPM_OK;
+}
+
+void icempm_mlp_destroy(icempm_mlp *m) {
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (float *p : {m->params, m->m1, m->m2, m->grad, m->part, m->loss})
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+const char *icempm_mlp_last_error(const icempm_mlp *m) { return m ? m->lastError.c_str() : ""; }
+
+int icempm_mlp_get_params(icempm_mlp *m, float *out) {
+    if (!m || !out) return ICEMPM_E_ARG;
+    MCHK(m, hipMemcpy(out, m->params, kParams * sizeof(float), hipMemcpyDeviceToHost), "mlp get");
+    return ICEMPM_OK;
+}
+
+const float *icempm_mlp_device_params(const icempm_mlp *m) { return m ? m->params : nullptr; }
+
+// X: n x 5 normalised inputs, T: n x 3 normalised targets (device pointers).
+// Gradient over rows [r0, r0+B) into *grad_out (host, may be NULL) and the
+// mean loss; lr > 0 also takes one Adam step.
+int icempm_mlp_step(icempm_mlp *m, const float *X, const float *T, int r0, int B, float lr,
+                    float *grad_out, float *loss_out) {
+    if (!m || !X || !T || B <= 0 || r0 < 0) return ICEMPM_E_ARG;
+    MCHK(m, icempm::launch_fill_i32(reinterpret_cast<int *>(m->loss), 1, 0, m->stream), "mlp step");
+    hipLaunchKernelGGL(k_mlp_grad, dim3(kGradBlocks), dim3(256), 0, m->stream, m->params, X, T, r0, B,
+                       m->part, m->loss);
+    MCHK(m, hipGetLastError(), "mlp grad");
+    const float b1 = 0.9f, b2 = 0.999f;
+    if (lr > 0) m->t += 1;
+    const float c1 = 1 - std::pow(b1, (float)(m->t > 0 ? m->t : 1));
+    const float c2 = 1 - std::pow(b2, (float)(m->t > 0 ? m->t : 1));
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, m->params, m->m1,
+                       m->m2, m->part, m->nparts, m->grad, lr, b1, b2, 1e-8f, c1, c2);
+    MCHK(m, hipGetLastError(), "mlp adam");
+    if (grad_out)
+        MCHK(m, hipMemcpyAsync(grad_out, m->grad, kParams * sizeof(float), hipMemcpyDeviceToHost, m->stream),
+             "mlp step");
+    float l = 0;
+    MCHK(m, hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream), "mlp step");
+    MCHK(m, hipStreamSynchronize(m->stream), "mlp step");
+    if (loss_out) *loss_out = l / (float)B;
+    return ICEMPM_OK;
+}
+
+// Y (device, n x 3) = MLP(X (device, n x 5))
+int icempm_mlp_infer(icempm_mlp *m, const float *X, int n, float *Y) {
+    if (!m || !X || !Y || n < 0) return ICEMPM_E_ARG;
+    if (n == 0) return ICEMPM_OK;
+    const int tiles = (n + 15) / 16;
+    int blocks = (tiles + 3) / 4;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, m->stream, m->params, X, n, Y);
+    MCHK(m, hipGetLastError(), "mlp infer");
+    MCHK(m, hipStreamSynchronize(m->stream), "mlp infer");
+    return ICEMPM_OK;
+}
+
+int icempm_device_alloc(size_t nbytes, void **ptr) {
+    if (!ptr) return ICEMPM_E_ARG;
+    *ptr = nullptr;
+    return hipMalloc(ptr, nbytes < 1 ? 1 : nbytes) == hipSuccess ? ICEMPM_OK : ICEMPM_E_HIP;
+}
+
+void icempm_device_free(void *ptr) {
+    if (ptr) (void)hipFree(ptr);
+}
+
+int icempm_copy_to_device(void *dst, const void *src, size_t nbytes) {
+    if (!dst || !src) return ICEMPM_E_ARG;
+    return hipMemcpy(dst, src, nbytes, hipMemcpyHostToDevice) == hipSuccess ? ICEMPM_OK : ICEMPM_E_HIP;
+}
+
+int icempm_copy_to_host(void *dst, const void *src, size_t nbytes) {
+    if (!dst || !src) return ICEMPM_E_ARG;
+    return hipMemcpy(dst, src, nbytes, hipMemcpyDeviceToHost) == hipSuccess ? ICEMPM_OK : ICEMPM_E_HIP;
+}
+
+// D (16 x 16, host) = A (16 x 16) * B (16 x 16) through one MFMA tile, for
+// the lane-layout test.
+int icempm_mfma_probe(const float *A, const float *B, float *D) {
+    float *d = nullptr;
+    hipError_t e = hipMalloc(&d, 3 * 256 * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(d, A, 256 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 256, B, 256 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, d, d + 256, d + 512);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(D, d + 512, 256 * sizeof(float), hipMemcpyDeviceToHost);
+    if (d) (void)hipFree(d);
+    return e == hipSuccess ? ICEMPM_OK : ICEMPM_E_HIP;
+}
+
+}  // extern "C"

@@ -190,6 +190,65 @@ int icempm_write_grid(icempm_ctx *ctx, const double *in);
 /* device bins (2*n_indenter_subdivisions doubles) to host */
 int icempm_read_bins(icempm_ctx *ctx, double *out);
 
+/* -- batches of independent cases (no counterpart) ----------------------
+ * Many indentation cases of uneven size advanced together: every substep is
+ * one grid-update launch and one particle-pass launch over all cases, and
+ * the substep loop can be captured in a HIP graph and replayed.  Cases are
+ * added from the reference SoA layout (16 rows x pitch doubles, as read back
+ * by icempm_read_points after LoadRawPoints); all cases share the time step;
+ * the material of a case must be uniform (GrainVariability = 0).
+ * Precision: ICEMPM_BATCH_F64 stores the point state in FP64;
+ * ICEMPM_BATCH_F32 stores velocity, Fe - I, Bp and Jp_inv in FP32 (positions
+ * and the grid stay FP64; all arithmetic is FP64).  The per-substep total
+ * indenter force of every case is recorded; per-bin forces are not. */
+typedef struct icempm_batch icempm_batch;
+enum { ICEMPM_BATCH_F64 = 0, ICEMPM_BATCH_F32 = 1 };
+int icempm_batch_create(icempm_batch **out, int precision);
+void icempm_batch_destroy(icempm_batch *b);
+const char *icempm_batch_last_error(const icempm_batch *b);
+/* returns the case index (>= 0) or -status */
+int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes, const double *soa,
+                          size_t pitch);
+int icempm_batch_finalize(icempm_batch *b);
+/* n substeps of every case from the batch's current time; use_graph != 0
+ * captures the loop once per n and replays it */
+int icempm_batch_run(icempm_batch *b, int n, int use_graph);
+/* device time of the last icempm_batch_run (HIP events on the batch stream) */
+int icempm_batch_last_ms(const icempm_batch *b, float *ms);
+int icempm_batch_num_points(const icempm_batch *b);
+/* total indenter force (fx, fy) of case c for each substep of the last run */
+int icempm_batch_force_history(icempm_batch *b, int c, double *out, int n);
+/* the state of case c in the reference SoA layout (P, Q and grain zero) */
+int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch);
+int icempm_batch_error_flag(icempm_batch *b, int *flag);
+
+/* -- the learned return map (no counterpart) -----------------------------
+ * A 5 -> 64 -> 64 -> 3 MLP (tanh hidden layers) with BF16 weights and
+ * activations on MFMA, FP32 accumulation, trained on the GPU with Adam.
+ * Parameters: icempm_mlp_num_params() floats, layout in csrc/mlp.hip.
+ * Inputs and targets are normalised by the caller; X is n x 5, T and Y are
+ * n x 3, all FP32 device pointers. */
+typedef struct icempm_mlp icempm_mlp;
+int icempm_mlp_num_params(void);
+int icempm_mlp_create(icempm_mlp **out, const float *init_params);
+void icempm_mlp_destroy(icempm_mlp *m);
+const char *icempm_mlp_last_error(const icempm_mlp *m);
+int icempm_mlp_get_params(icempm_mlp *m, float *out);
+const float *icempm_mlp_device_params(const icempm_mlp *m);
+/* gradient of 1/(2B) sum |MLP(X) - T|^2 over rows [r0, r0+B) (host copy in
+ * grad_out if not NULL) and the mean loss; lr > 0 also applies one Adam
+ * step (beta 0.9, 0.999, eps 1e-8) */
+int icempm_mlp_step(icempm_mlp *m, const float *X, const float *T, int r0, int B, float lr,
+                    float *grad_out, float *loss_out);
+int icempm_mlp_infer(icempm_mlp *m, const float *X, int n, float *Y);
+/* plain device memory for training data and tests */
+int icempm_device_alloc(size_t nbytes, void **ptr);
+void icempm_device_free(void *ptr);
+int icempm_copy_to_device(void *dst, const void *src, size_t nbytes);
+int icempm_copy_to_host(void *dst, const void *src, size_t nbytes);
+/* one 16x16x16 BF16 MFMA tile on host matrices (row-major), for layout tests */
+int icempm_mfma_probe(const float *A, const float *B, float *D);
+
 #ifdef __cplusplus
 }
 #endif

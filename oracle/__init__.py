@@ -58,6 +58,8 @@ def lib() -> ctypes.CDLL:
     L.oracle_cell_keys_f32.argtypes = [POINTER(c_float), c_int, c_double, c_int, ip]
     L.oracle_cell_keys_soa.argtypes = [dp, c_size_t, c_int, c_double, c_int, ip]
     L.oracle_stable_order.argtypes = [ip, c_int, ip]
+    L.oracle_mlp_loss_grad.argtypes = [dp, dp, dp, c_int, dp, dp]
+    L.oracle_mlp_loss_grad.restype = c_double
     _lib = L
     return L
 
@@ -217,3 +219,19 @@ def ingest(prms, xy: np.ndarray, grains: np.ndarray, volume: float, pitch: int) 
     pts[8, :n] = 1.0
     pts[11, :n] = 1.0
     return pts
+
+
+# -- learned return map --------------------------------------------------------
+MLP_PARAMS = 6288
+
+
+def mlp_loss_grad(params, X, T, want_grad=True):
+    """FP64 loss, gradient and outputs of the MLP (layout as in csrc/mlp.hip)."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    B = X.shape[0]
+    g = np.zeros(MLP_PARAMS) if want_grad else None
+    Y = np.zeros((B, 3))
+    L = lib().oracle_mlp_loss_grad(_dp(P), _dp(X), _dp(T), B, _dp(g) if want_grad else None, _dp(Y))
+    return L, g, Y

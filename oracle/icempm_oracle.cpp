@@ -700,6 +700,70 @@ int oracle_run(void *prms, double *pts, double *grid, double *bins, int nsub, do
     return err;
 }
 
+// ---- learned return map: FP64 restatement of the MLP of csrc/mlp.hip ----
+// params: W1[64][16] b1[64] W2[64][64] b2[64] W3[16][64] b3[16] (6288);
+// X is B x 5, T is B x 3.  Returns L = 1/(2B) sum (y - t)^2 over the 3
+// outputs; grad (6288, may be NULL) receives dL/dparams; Y (B x 3, may be
+// NULL) the outputs.
+double oracle_mlp_loss_grad(const double *P, const double *X, const double *T, int B,
+                            double *grad, double *Y) {
+    const int oW1 = 0, oB1 = 1024, oW2 = 1088, oB2 = 5184, oW3 = 5248, oB3 = 6272, NP = 6288;
+    if (grad) std::fill(grad, grad + NP, 0.0);
+    double L = 0;
+    double x[16], h1[64], h2[64], y[16], d2[64], d1[64], dy[16];
+    for (int r = 0; r < B; ++r) {
+        for (int k = 0; k < 16; ++k) x[k] = k < 5 ? X[5 * r + k] : 0.0;
+        for (int n = 0; n < 64; ++n) {
+            double a = P[oB1 + n];
+            for (int k = 0; k < 16; ++k) a += P[oW1 + n * 16 + k] * x[k];
+            h1[n] = std::tanh(a);
+        }
+        for (int n = 0; n < 64; ++n) {
+            double a = P[oB2 + n];
+            for (int k = 0; k < 64; ++k) a += P[oW2 + n * 64 + k] * h1[k];
+            h2[n] = std::tanh(a);
+        }
+        for (int o = 0; o < 16; ++o) {
+            double a = P[oB3 + o];
+            for (int k = 0; k < 64; ++k) a += P[oW3 + o * 64 + k] * h2[k];
+            y[o] = a;
+        }
+        for (int o = 0; o < 16; ++o) {
+            dy[o] = 0;
+            if (o < 3) {
+                const double d = y[o] - T[3 * r + o];
+                L += 0.5 * d * d / B;
+                dy[o] = d / B;
+                if (Y) Y[3 * r + o] = y[o];
+            }
+        }
+        if (!grad) continue;
+        for (int o = 0; o < 16; ++o) {
+            grad[oB3 + o] += dy[o];
+            for (int k = 0; k < 64; ++k) grad[oW3 + o * 64 + k] += dy[o] * h2[k];
+        }
+        for (int k = 0; k < 64; ++k) {
+            double a = 0;
+            for (int o = 0; o < 16; ++o) a += dy[o] * P[oW3 + o * 64 + k];
+            d2[k] = a * (1 - h2[k] * h2[k]);
+        }
+        for (int n = 0; n < 64; ++n) {
+            grad[oB2 + n] += d2[n];
+            for (int k = 0; k < 64; ++k) grad[oW2 + n * 64 + k] += d2[n] * h1[k];
+        }
+        for (int k = 0; k < 64; ++k) {
+            double a = 0;
+            for (int n = 0; n < 64; ++n) a += d2[n] * P[oW2 + n * 64 + k];
+            d1[k] = a * (1 - h1[k] * h1[k]);
+        }
+        for (int n = 0; n < 64; ++n) {
+            grad[oB1 + n] += d1[n];
+            for (int k = 0; k < 16; ++k) grad[oW1 + n * 16 + k] += d1[n] * x[k];
+        }
+    }
+    return L;
+}
+
 void oracle_cell_keys_f32(const float *xy, int n, double hinv, int gridx, int *keys) {
     for (int i = 0; i < n; ++i) {
         float ax = xy[2 * i], ay = xy[2 * i + 1];

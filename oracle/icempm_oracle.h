@@ -82,6 +82,10 @@ void oracle_cell_keys_soa(const double *pts, size_t pitch, int n, double hinv,
 /* stable ascending order of keys: perm[k] = index of k-th smallest */
 void oracle_stable_order(const int *keys, int n, int *perm);
 
+/* ---- learned return map: FP64 forward/backward of the MLP of csrc/mlp.hip */
+double oracle_mlp_loss_grad(const double *params, const double *X, const double *T, int B,
+                            double *grad, double *Y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,68 @@
+"""Batches of independent cases (one launch per kernel per substep for all
+cases, HIP graph replay) against the CPU oracle run case by case."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from icempm_amd import SimParams
+from icempm_amd._lib import BATCH_F32, BATCH_F64
+from icempm_amd.batch import batch_cases, build_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    n = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / n if n > 0 else np.linalg.norm(a - b)
+
+
+def oracle_case(p0, soa0, nsub):
+    p = SimParams()
+    ctypes.memmove(ctypes.byref(p), ctypes.byref(p0), ctypes.sizeof(SimParams))
+    pts = soa0.copy()
+    grid = oracle.new_grid(p)
+    bins = np.zeros(2 * p.n_indenter_subdivisions)
+    hist, err = oracle.run(p, pts, grid, bins, nsub)
+    assert err == 0
+    return hist, pts
+
+
+@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 1e-9)],
+                         ids=["f64", "f32"])
+def test_batch_matches_oracle_per_case(gpu_available, precision, tol_f, tol_x):
+    print("batch", precision, flush=True)
+    cases = batch_cases(3, 2000, 6000, seed=11)
+    b = build_batch(cases, precision)
+    nsub = 150
+    b.run(nsub, use_graph=True)
+    assert b.error_flag() == 0
+    for c, case in enumerate(cases):
+        p0, soa0 = b.initial[c]
+        href, pref = oracle_case(p0, soa0, nsub)
+        hist = b.force_history(c, nsub)
+        n = case.n_points
+        dev = b.points(c)
+        assert np.abs(href).sum() > 0
+        assert rel_l2(hist, href) <= tol_f, (c, rel_l2(hist, href))
+        assert np.abs(dev[4:6, :n] - pref[4:6, :n]).max() <= tol_x
+        assert np.array_equal(dev[0].view(np.uint8)[:n], pref[0].view(np.uint8)[:n])
+    b.close()
+
+
+def test_batch_graph_replay_matches_stream_run(gpu_available):
+    print("batch graph", flush=True)
+    cases = batch_cases(4, 1000, 4000, seed=12)
+    g = build_batch(cases, BATCH_F64)
+    s = build_batch(cases, BATCH_F64)
+    for _ in range(3):                       # replays with host work between them
+        g.run(40, use_graph=True)
+        s.run(40, use_graph=False)
+        for c in range(len(cases)):
+            assert rel_l2(g.force_history(c, 40), s.force_history(c, 40)) <= 1e-9
+    for c, case in enumerate(cases):
+        n = case.n_points
+        assert np.abs(g.points(c)[4:6, :n] - s.points(c)[4:6, :n]).max() <= 1e-12
+    g.close()
+    s.close()

@@ -1,0 +1,72 @@
+"""The learned-map MLP kernels: MFMA lane layout, gradients against the FP64
+oracle, inference, and that Adam training reduces the loss."""
+import numpy as np
+import pytest
+
+import oracle
+from icempm_amd.learned import MLP, DeviceArray, init_params, mfma_probe
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_tile_layout(gpu_available):
+    print("mfma probe", flush=True)
+    rng = np.random.default_rng(1)
+    A = rng.integers(-8, 8, size=(16, 16)).astype(np.float32)   # exact in bf16
+    B = rng.integers(-8, 8, size=(16, 16)).astype(np.float32)
+    D = mfma_probe(A, B)
+    assert np.array_equal(D, A @ B)
+
+
+def data(rng, n):
+    X = rng.normal(size=(n, 5)).astype(np.float32)
+    T = np.stack([np.tanh(X[:, 0] + 0.5 * X[:, 1]), X[:, 2] * X[:, 3] * 0.3,
+                  np.sin(X[:, 4])], axis=1).astype(np.float32)
+    return X, T
+
+
+def test_gradient_against_fp64_oracle(gpu_available):
+    print("mlp gradient", flush=True)
+    rng = np.random.default_rng(2)
+    P = init_params(3)
+    X, T = data(rng, 1000)
+    m = MLP(P)
+    dX, dT = DeviceArray(X), DeviceArray(T)
+    loss, g = m.step(dX, dT, 0, 1000, 0.0, want_grad=True)
+    L, gref, _ = oracle.mlp_loss_grad(P.astype(np.float64), X, T)
+    assert loss == pytest.approx(L, rel=2e-2)
+    # BF16 operands: agreement to about 1e-2 of the gradient's norm
+    assert np.linalg.norm(g - gref) / np.linalg.norm(gref) <= 3e-2
+    for o, n in ((0, 1024), (1088, 4096), (5248, 1024)):
+        blk, ref = g[o:o + n], gref[o:o + n]
+        assert np.linalg.norm(blk - ref) / np.linalg.norm(ref) <= 5e-2
+    # the parameters did not move (lr = 0)
+    assert np.array_equal(m.params(), P)
+    m.close()
+
+
+def test_inference_against_fp64_oracle(gpu_available):
+    print("mlp infer", flush=True)
+    rng = np.random.default_rng(4)
+    P = init_params(5)
+    X, T = data(rng, 777)
+    m = MLP(P)
+    Y = m.infer(DeviceArray(X), 777)
+    _, _, Yref = oracle.mlp_loss_grad(P.astype(np.float64), X, T, want_grad=False)
+    assert np.abs(Y - Yref).max() <= 2e-2 * max(1.0, np.abs(Yref).max())
+    m.close()
+
+
+def test_training_reduces_loss(gpu_available):
+    print("mlp train", flush=True)
+    rng = np.random.default_rng(6)
+    X, T = data(rng, 65536)
+    m = MLP(init_params(7))
+    dX, dT = DeviceArray(X), DeviceArray(T)
+    first, _ = m.step(dX, dT, 0, 65536, 0.0)
+    for it in range(400):
+        r0 = (it * 4096) % 65536
+        m.step(dX, dT, r0, 4096, 3e-3)
+    last, _ = m.step(dX, dT, 0, 65536, 0.0)
+    assert last < first / 10, (first, last)
+    m.close()
