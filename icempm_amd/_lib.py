@@ -25,6 +25,9 @@ MODE_ORDERED = 1
 TUNE_PASS_KIND = 0
 BATCH_F64 = 0
 BATCH_F32 = 1
+MAP_EXACT = 0
+MAP_HARVEST = 1
+MAP_LEARNED = 2
 
 
 class IcempmError(RuntimeError):
@@ -124,6 +127,7 @@ HIP_EXPORTS = [
     "icempm_batch_last_error", "icempm_batch_add_case", "icempm_batch_finalize",
     "icempm_batch_run", "icempm_batch_last_ms", "icempm_batch_num_points",
     "icempm_batch_force_history", "icempm_batch_read_points", "icempm_batch_error_flag",
+    "icempm_batch_set_return_map", "icempm_batch_read_harvest", "icempm_batch_map_stats",
     "icempm_mlp_num_params", "icempm_mlp_create", "icempm_mlp_destroy", "icempm_mlp_last_error",
     "icempm_mlp_get_params", "icempm_mlp_device_params", "icempm_mlp_step", "icempm_mlp_infer",
     "icempm_mfma_probe", "icempm_device_alloc", "icempm_device_free", "icempm_copy_to_device",
@@ -199,6 +203,11 @@ def hip():
     _sig(L, "icempm_batch_force_history", c_int, P, c_int, POINTER(c_double), c_int)
     _sig(L, "icempm_batch_read_points", c_int, P, c_int, POINTER(c_double), c_size_t)
     _sig(L, "icempm_batch_error_flag", c_int, P, POINTER(c_int))
+    _sig(L, "icempm_batch_set_return_map", c_int, P, c_int, POINTER(c_float), POINTER(c_float),
+         c_double, c_int)
+    _sig(L, "icempm_batch_read_harvest", c_int, P, POINTER(c_float), POINTER(c_float), c_int,
+         POINTER(c_uint64))
+    _sig(L, "icempm_batch_map_stats", c_int, P, POINTER(c_uint64), c_int)
     fp = POINTER(c_float)
     _sig(L, "icempm_mlp_num_params", c_int)
     _sig(L, "icempm_mlp_create", c_int, POINTER(c_void_p), fp)

@@ -7,12 +7,13 @@ from __future__ import annotations
 
 import ctypes
 import math
-from ctypes import POINTER, byref, c_double, c_float, c_int, c_void_p
+from ctypes import POINTER, byref, c_double, c_float, c_int, c_uint64, c_void_p
 
 import numpy as np
 
 from . import _lib
-from ._lib import BATCH_F32, BATCH_F64, IcempmError, SimParams
+from ._lib import (BATCH_F32, BATCH_F64, MAP_EXACT, MAP_HARVEST, MAP_LEARNED, IcempmError,
+                   SimParams)
 from .model import MODE_FAST, IndentationCase, Model, nacc_strengths
 
 
@@ -111,6 +112,46 @@ class Batch:
         f = c_int(0)
         self._check(self._H.icempm_batch_error_flag(self._b, byref(f)))
         return f.value
+
+    # -- return map of crushed points ------------------------------------------
+    def set_exact(self) -> None:
+        self._check(self._H.icempm_batch_set_return_map(self._b, MAP_EXACT, None, None, 0.0, 0))
+
+    def set_harvest(self, capacity: int) -> None:
+        """Exact map, recording up to ``capacity`` (features, targets) pairs."""
+        self._check(self._H.icempm_batch_set_return_map(self._b, MAP_HARVEST, None, None, 0.0,
+                                                        int(capacity)))
+        self._hcap = int(capacity)
+
+    def set_learned(self, params: np.ndarray, norm: np.ndarray, tol: float = 1e-4) -> None:
+        """MLP proposal accepted where the yield residual is <= tol, else the
+        exact map.  norm: 16 floats (input mean 5, input scale 5, output mean
+        3, output scale 3)."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        if p.shape != (6288,) or nm.shape != (16,):
+            raise ValueError("params must have 6288 and norm 16 entries")
+        fp = POINTER(c_float)
+        self._check(self._H.icempm_batch_set_return_map(self._b, MAP_LEARNED, p.ctypes.data_as(fp),
+                                                        nm.ctypes.data_as(fp), float(tol), 0))
+
+    def read_harvest(self, cap: int) -> tuple[np.ndarray, np.ndarray, int]:
+        """(X (n x 5), T (n x 3), pairs offered); n <= cap and <= the capacity."""
+        X = np.zeros((max(cap, 0), 5), dtype=np.float32)
+        T = np.zeros((max(cap, 0), 3), dtype=np.float32)
+        tot = c_uint64(0)
+        fp = POINTER(c_float)
+        self._check(self._H.icempm_batch_read_harvest(self._b, X.ctypes.data_as(fp), T.ctypes.data_as(fp),
+                                                      int(cap), byref(tot)))
+        n = min(int(tot.value), cap, getattr(self, "_hcap", 0))
+        return X[:n], T[:n], int(tot.value)
+
+    def map_stats(self, reset: bool = False) -> tuple[int, int, int]:
+        """(points accepted from the MLP, points that fell back to the exact
+        map, crushed points inside the surface), summed over substeps."""
+        out = (c_uint64 * 3)()
+        self._check(self._H.icempm_batch_map_stats(self._b, out, 1 if reset else 0))
+        return int(out[0]), int(out[1]), int(out[2])
 
 
 def build_batch(cases: list[IndentationCase], precision: int = BATCH_F64) -> Batch:

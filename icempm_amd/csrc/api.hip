@@ -104,6 +104,8 @@ void hfree(T *&p) {
     p = nullptr;
 }
 
+void grain_row(const icy::SimParams &P, int gi, double *row);
+
 KParams make_kparams(const icempm_ctx *c) {
     const icy::SimParams &P = c->prms;
     KParams k{};
@@ -136,6 +138,8 @@ KParams make_kparams(const icempm_ctx *c) {
     k.grainVar = P.GrainVariability;
     k.tanPhi = P.DP_tan_phi;
     k.dpThr = P.DP_threshold_p;
+    k.uniformGrain = P.GrainVariability == 0 ? 1 : 0;
+    if (k.uniformGrain) grain_row(P, 0, k.g0);
     return k;
 }
 
@@ -144,29 +148,30 @@ KParams make_kparams(const icempm_ctx *c) {
 // (gpu_implementation5.cu:97-141), so that the table lookup returns exactly
 // what the per-particle formula would (this file is built without FMA
 // contraction; sqrt and division are IEEE on host and device alike).
+void grain_row(const icy::SimParams &P, int gi, double *row) {
+    const short grain = (short)gi;
+    const double var1 = 1.0 + P.GrainVariability * 0.05 * (-10 + grain % 21);
+    const double var2 = 1.0 + P.GrainVariability * 0.033 * (-15 + (grain + 3) % 30);
+    const double var3 = 1.0 + P.GrainVariability * 0.1 * (-10 + (grain + 4) % 11);
+    const double pmax = P.IceCompressiveStrength * var1;
+    const double pmin = -P.IceTensileStrength * var2;
+    const double qmax = P.IceShearStrength * var3;
+    const double beta = -pmin / pmax;
+    const double a = pmax * (1 + beta);
+    const double mSq = (4 * qmax * qmax * (1 + 2 * beta)) / (a * a);
+    const double pmin2 = -3e6;
+    const double q0 = 2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin);
+    const double slope = -q0 / pmin2;
+    row[kGPmax] = pmax;
+    row[kGMsq] = mSq;
+    row[kGBetaPmax] = beta * pmax;
+    row[kGOnePlus2Beta] = 1. + 2. * beta;
+    row[kGSlope] = slope;
+}
+
 void grain_table(const icy::SimParams &P, std::vector<double> &t) {
     t.assign((size_t)kGrainEntries * kGrainFields, 0.0);
-    for (int gi = -32768; gi < 32768; ++gi) {
-        const short grain = (short)gi;
-        const double var1 = 1.0 + P.GrainVariability * 0.05 * (-10 + grain % 21);
-        const double var2 = 1.0 + P.GrainVariability * 0.033 * (-15 + (grain + 3) % 30);
-        const double var3 = 1.0 + P.GrainVariability * 0.1 * (-10 + (grain + 4) % 11);
-        const double pmax = P.IceCompressiveStrength * var1;
-        const double pmin = -P.IceTensileStrength * var2;
-        const double qmax = P.IceShearStrength * var3;
-        const double beta = -pmin / pmax;
-        const double a = pmax * (1 + beta);
-        const double mSq = (4 * qmax * qmax * (1 + 2 * beta)) / (a * a);
-        const double pmin2 = -3e6;
-        const double q0 = 2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin);
-        const double slope = -q0 / pmin2;
-        double *row = t.data() + (size_t)(gi + 32768) * kGrainFields;
-        row[kGPmax] = pmax;
-        row[kGMsq] = mSq;
-        row[kGBetaPmax] = beta * pmax;
-        row[kGOnePlus2Beta] = 1. + 2. * beta;
-        row[kGSlope] = slope;
-    }
+    for (int gi = -32768; gi < 32768; ++gi) grain_row(P, gi, t.data() + (size_t)(gi + 32768) * kGrainFields);
 }
 
 // Launch preconditions shared by every substep operation.

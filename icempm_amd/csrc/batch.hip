@@ -16,6 +16,7 @@
 // All arithmetic is FP64 in both variants (the return map included); the F32
 // variant only stores less: 93 B per point and substep instead of 164 B.
 // Material is uniform per case (one set of NACC constants per case).
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -25,6 +26,7 @@
 #include "../../include/icempm.h"
 #include "../host/parameters_sim.h"
 #include "internal.h"
+#include "mlp_device.h"
 
 #pragma clang fp contract(fast)
 
@@ -77,15 +79,85 @@ static __device__ __forceinline__ void rot3(bool s, T &a0, T &a1, T &a2) {
     a0 = t0; a1 = t1; a2 = t2;
 }
 
+// Return map of crushed points: the exact Drucker-Prager map, the exact map
+// while recording (trial, projected) pairs, or the learned map with the
+// exact one as fallback.
+enum { kMapExact = 0, kMapHarvest = 1, kMapLearned = 2 };
+
+struct LearnArgs {
+    const unsigned short *wbf;   // MLP weights, BF16 bits
+    const float *pf;             // MLP parameters, FP32 (biases)
+    mlpdev::Norm norm;
+    double tol;                  // accepted when the yield residual is <= tol
+    float *hx, *ht;              // harvest: inputs (n x 5), targets (n x 3)
+    unsigned long long *hcount;
+    int hcap;
+    unsigned long long *stats;   // [accepted, fell back, inside the surface]
+};
+
+// The branch of the exact map (Wolper_Drucker_Prager, gpu_implementation5.cu:48-94)
+// a crushed trial state falls in, decided exactly: the tension branch, the
+// shear projection onto q = q_lim, or inside the surface (the map is the
+// identity there).  qlim is set for the shear branch.
+enum { kBrInside = 0, kBrTension = 1, kBrShear = 2 };
+static __device__ __forceinline__ int map_branch(const KParams &k, const Trial &r, double Jp, double &qlim) {
+    qlim = 0;
+    if (r.p < -k.dpThr || Jp < 1) return kBrTension;
+    qlim = (r.p > k.ics) ? 0.0 : fmin((r.p + k.dpThr) * k.tanPhi, k.iss);
+    return (r.q >= qlim) ? kBrShear : kBrInside;
+}
+
+// Raw features of a crushed point for the map: log s0, log s1, log Jp_inv,
+// a hardening variable the reference does not have (0), and the yield-state
+// flag: 1 in the tension branch, 0 in the shear branch.
+static __device__ __forceinline__ void map_features(const Trial &r, double Jp, int br, double f[5]) {
+    f[0] = log(r.svd.s0);
+    f[1] = log(r.svd.s1);
+    f[2] = log(Jp);
+    f[3] = 0.0;
+    f[4] = (br == kBrTension) ? 1.0 : 0.0;
+}
+
+// Yield residual of a proposed projected state (s0, s1, Jpn), as stress
+// errors relative to the case's strengths:
+//  tension  the state must be isotropic at p = -p_thr:
+//           max(|p(s) + p_thr| / pmax, q(s) / qmax, |Jpn / Jp_exact - 1|)
+//  shear    the deviator (scaled with the trial J, as the map scales it)
+//           must sit at q_lim and the mean of s^2 must be kept:
+//           max(|q~(s) - q_lim| / qmax, kappa |mean s^2 - mean s_tr^2| / pmax)
+static __device__ __forceinline__ double yield_residual(const KParams &k, const Trial &r, double Jp, int br,
+                                                        double qlim, double s0, double s1, double Jpn) {
+    const double a = s0 * s0, b = s1 * s1;
+    if (br == kBrTension) {
+        const double Je_new = sqrt(-2. * -k.dpThr / k.kappa + 1.);
+        const double Jn = s0 * s1;
+        const double pn = -(k.kappa / 2.) * (Jn * Jn - 1.);
+        const double qn = (k.mu / Jn) * fabs(a - b);
+        double res = fabs(pn + k.dpThr) / k.ics;
+        res = fmax(res, qn / k.iss);
+        res = fmax(res, fabs(Jpn / (Jp * (Je_new / r.Je)) - 1.));
+        return res;
+    }
+    const double qt = (k.mu / r.Je) * fabs(a - b);
+    const double dm = 0.5 * (a + b) - 0.5 * (r.ss0 + r.ss1);
+    return fmax(fabs(qt - qlim) / k.iss, k.kappa * fabs(dm) / k.ics);
+}
+
+// Bytes of shared memory per wavefront: the P2G window, which the learned
+// map's inference scratch reuses during G2P.
+constexpr int kWaveWinBytes = 3 * kWY * kWS * 8;
+constexpr int kWaveBytes = kWaveWinBytes > (int)sizeof(mlpdev::InferScratch) ? kWaveWinBytes
+                                                                             : (int)sizeof(mlpdev::InferScratch);
+
 // Fused particle pass over all cases (G2P of substep k and/or P2G of k+1).
-template <bool F32, bool G2P, bool P2G>
+template <bool F32, bool G2P, bool P2G, int RMAP>
 __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const CaseParams *__restrict__ cases,
                                                          const int *__restrict__ chunkStart,
                                                          const int *__restrict__ chunkCase,
                                                          int nChunks, const double *__restrict__ gridV,
                                                          double *__restrict__ gridOut, size_t gpitch,
-                                                         int *__restrict__ err) {
-    __shared__ double wwin[4][3][kWY * kWS];
+                                                         int *__restrict__ err, LearnArgs la) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[4][RMAP == kMapLearned ? kWaveBytes : kWaveWinBytes];
     const int per = (nChunks + 7) >> 3;
     const int c = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (c >= nChunks) return;
@@ -126,6 +198,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         const bool ok = valid && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
                         stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
         if (valid && !ok) flag_error(k);
+        Trial r{};
+        unsigned char crushed0 = crushed;
+        double Jp0 = Jp;
         if (ok) {
             double n0[9], n1[9];
             if (bx + 2 < gx && by + 2 < gy) {
@@ -180,9 +255,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             const double n00 = M00 * F00 + M01 * F10, n01 = M00 * F01 + M01 * F11;
             const double n10 = M10 * F00 + M11 * F10, n11 = M10 * F01 + M11 * F11;
             F00 = n00; F01 = n01; F10 = n10; F11 = n11;
-            const Trial r = compute_pq(F00, F01, F10, F11, cp.mu, cp.kappa);
-            const unsigned char crushed0 = crushed;
-            const double Jp0 = Jp;
+            r = compute_pq(F00, F01, F10, F11, cp.mu, cp.kappa);
+            crushed0 = crushed;
+            Jp0 = Jp;
             if (crushed == 0) {
                 bool f;
                 if (r.p < 0) {
@@ -192,9 +267,97 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
                 }
                 crushed = f ? 1 : 0;
             }
-            if (crushed == 1) dp_return(k, r, F00, F01, F10, F11, Jp);
             vx = v0; vy = v1;
             B00 = b00; B01 = b01; B10 = b10; B11 = b11;
+        }
+        // ---- return map (convergent: the learned map runs wavefront-wide) ----
+        const bool crushedOk = ok && crushed == 1;
+        if constexpr (RMAP == kMapExact) {
+            if (crushedOk) dp_return(k, r, F00, F01, F10, F11, Jp);
+        } else if constexpr (RMAP == kMapHarvest) {
+            double f[5] = {0, 0, 0, 0, 0}, t[3] = {0, 0, 0};
+            bool usable = false;
+            if (crushedOk) {
+                double qlim;
+                const int br = map_branch(k, r, Jp, qlim);
+                map_features(r, Jp, br, f);
+                dp_return(k, r, F00, F01, F10, F11, Jp);
+                if (br != kBrInside) {
+                    const Trial rn = compute_pq(F00, F01, F10, F11, cp.mu, cp.kappa);
+                    usable = r.svd.s1 > 0 && rn.svd.s1 > 0;
+                    t[0] = log(rn.svd.s0) - f[0];
+                    t[1] = log(rn.svd.s1) - f[1];
+                    t[2] = log(Jp) - f[2];
+                }
+            }
+            // one counter update per wavefront
+            const unsigned long long um = __ballot(usable);
+            if (um) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(la.hcount, (unsigned long long)__popcll(um));
+                base = __shfl(base, 0, 64);
+                const unsigned long long slot = base + (unsigned long long)__popcll(um & ((1ull << lane) - 1ull));
+                if (usable && slot < (unsigned long long)la.hcap) {
+                    for (int q = 0; q < 5; ++q) la.hx[slot * 5 + q] = (float)f[q];
+                    for (int q = 0; q < 3; ++q) la.ht[slot * 3 + q] = (float)t[q];
+                }
+            }
+        } else {
+            // the branch is decided exactly; inside the surface the map is
+            // the identity, and the network proposes the projected state of
+            // the two other branches
+            double qlim = 0;
+            const int br = crushedOk ? map_branch(k, r, Jp, qlim) : kBrInside;
+            const bool proj = crushedOk && br != kBrInside;
+            const bool need = proj && r.svd.s1 > 0;
+            const unsigned long long mask = __ballot(need);
+            double pred[3] = {0, 0, 0};
+            if (mask) {
+                mlpdev::InferScratch &isc = *reinterpret_cast<mlpdev::InferScratch *>(smem[wv]);
+                const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+                const int cnt = __popcll(mask);
+                float xin[5] = {0, 0, 0, 0, 0};
+                if (need) {
+                    double f[5];
+                    map_features(r, Jp, br, f);
+                    for (int q = 0; q < 5; ++q)
+                        xin[q] = (float)((f[q] - la.norm.in_mean[q]) / la.norm.in_scale[q]);
+                }
+                for (int base = 0; base < cnt; base += 16) {
+                    for (int e = lane; e < 16 * mlpdev::kIn; e += 64) isc.x[e / mlpdev::kIn][e % mlpdev::kIn] = 0;
+                    mlpdev::wave_fence();
+                    if (need && rank >= base && rank < base + 16)
+                        for (int q = 0; q < 5; ++q) isc.x[rank - base][q] = mlpdev::to_bf16(xin[q]);
+                    mlpdev::wave_fence();
+                    mlpdev::infer16(isc, la.wbf, la.pf);
+                    if (need && rank >= base && rank < base + 16)
+                        for (int q = 0; q < 3; ++q)
+                            pred[q] = (double)isc.y[rank - base][q] * la.norm.out_scale[q] + la.norm.out_mean[q];
+                    mlpdev::wave_fence();
+                }
+            }
+            bool accepted = false;
+            if (need) {
+                const double s0 = exp(log(r.svd.s0) + pred[0]);
+                const double s1 = exp(log(r.svd.s1) + pred[1]);
+                // the shear branch leaves Jp_inv unchanged
+                const double Jpn = (br == kBrTension) ? Jp * exp(pred[2]) : Jp;
+                if (yield_residual(k, r, Jp, br, qlim, s0, s1, Jpn) <= la.tol) {
+                    usvt(r.svd, s0, s1, F00, F01, F10, F11);
+                    Jp = Jpn;
+                    accepted = true;
+                }
+            }
+            if (proj && !accepted) dp_return(k, r, F00, F01, F10, F11, Jp);
+            const unsigned long long acc = __ballot(accepted), fb = __ballot(proj && !accepted),
+                                     ins = __ballot(crushedOk && !proj);
+            if (lane == 0 && (acc | fb | ins)) {
+                if (acc) atomicAdd(&la.stats[0], (unsigned long long)__popcll(acc));
+                if (fb) atomicAdd(&la.stats[1], (unsigned long long)__popcll(fb));
+                if (ins) atomicAdd(&la.stats[2], (unsigned long long)__popcll(ins));
+            }
+        }
+        if (ok) {
             S.pos[pt] = px;
             S.pos[pp + pt] = py;
             R[pt] = (V)vx;
@@ -246,7 +409,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         if (valid && !ok) flag_error(k);
         const int ox = wmin(ok ? bx : INT_MAX);
         const int oy = wmin(ok ? by : INT_MAX);
-        double *W0 = wwin[wv][0], *W1 = wwin[wv][1], *W2 = wwin[wv][2];
+        double *W0 = reinterpret_cast<double *>(smem[wv]);
+        double *W1 = W0 + kWY * kWS, *W2 = W1 + kWY * kWS;
         for (int e = lane; e < kWS * kWY; e += 64) {
             W0[e] = 0;
             W1[e] = 0;
@@ -467,6 +631,17 @@ struct icempm_batch {
     double t = 0;
     int step = 0;
 
+    // return map of crushed points
+    int rmap = kMapExact;
+    unsigned short *mlpW = nullptr;   // MLP parameters as BF16 bits
+    float *mlpP = nullptr;            // and as FP32
+    mlpdev::Norm norm{};
+    double tol = 1e-4;
+    float *hx = nullptr, *ht = nullptr;
+    unsigned long long *hcount = nullptr;
+    int hcap = 0;
+    unsigned long long *stats = nullptr;
+
     hipGraphExec_t graph = nullptr;
     int graphSteps = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -492,27 +667,52 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+template <bool F32, bool G2P, bool P2G, int RMAP>
+void launch_one(icempm_batch *b, unsigned nb, const double *gV, double *gO, const LearnArgs &la) {
+    Rows<F32> S{b->pos, reinterpret_cast<typename Rows<F32>::V *>(b->rest), b->crushed, b->pitch};
+    hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream, S,
+                       b->cases, b->chunkStart, b->chunkCase, b->nChunks, gV, gO, b->gpitch, b->err, la);
+}
+
+template <bool F32, int RMAP>
+void launch_map(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const double *gV, double *gO,
+                const LearnArgs &la) {
+    if (g2p && p2g)
+        launch_one<F32, true, true, RMAP>(b, nb, gV, gO, la);
+    else if (g2p)
+        launch_one<F32, true, false, RMAP>(b, nb, gV, gO, la);
+    else
+        launch_one<F32, false, true, kMapExact>(b, nb, gV, gO, la);
+}
+
+template <bool F32>
+void launch_prec(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const double *gV, double *gO,
+                 const LearnArgs &la) {
+    if (b->rmap == kMapHarvest)
+        launch_map<F32, kMapHarvest>(b, nb, g2p, p2g, gV, gO, la);
+    else if (b->rmap == kMapLearned)
+        launch_map<F32, kMapLearned>(b, nb, g2p, p2g, gV, gO, la);
+    else
+        launch_map<F32, kMapExact>(b, nb, g2p, p2g, gV, gO, la);
+}
+
 hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const double *gV, double *gO) {
     const unsigned nb = (unsigned)(((b->nChunks + 7) / 8) * 8);
     if (b->nChunks <= 0) return hipSuccess;
-#define BL(F32)                                                                                    \
-    do {                                                                                           \
-        Rows<F32> S{b->pos, reinterpret_cast<typename Rows<F32>::V *>(b->rest), b->crushed, b->pitch}; \
-        if (g2p && p2g)                                                                            \
-            hipLaunchKernelGGL((k_batch_particles<F32, true, true>), dim3(nb), dim3(256), 0, b->stream, S, \
-                               b->cases, b->chunkStart, b->chunkCase, b->nChunks, gV, gO, b->gpitch, b->err); \
-        else if (g2p)                                                                              \
-            hipLaunchKernelGGL((k_batch_particles<F32, true, false>), dim3(nb), dim3(256), 0, b->stream, S, \
-                               b->cases, b->chunkStart, b->chunkCase, b->nChunks, gV, gO, b->gpitch, b->err); \
-        else                                                                                       \
-            hipLaunchKernelGGL((k_batch_particles<F32, false, true>), dim3(nb), dim3(256), 0, b->stream, S, \
-                               b->cases, b->chunkStart, b->chunkCase, b->nChunks, gV, gO, b->gpitch, b->err); \
-    } while (0)
+    LearnArgs la{};
+    la.wbf = b->mlpW;
+    la.pf = b->mlpP;
+    la.norm = b->norm;
+    la.tol = b->tol;
+    la.hx = b->hx;
+    la.ht = b->ht;
+    la.hcount = b->hcount;
+    la.hcap = b->hcap;
+    la.stats = b->stats;
     if (b->precision == ICEMPM_BATCH_F32)
-        BL(true);
+        launch_prec<true>(b, nb, g2p, p2g, gV, gO, la);
     else
-        BL(false);
-#undef BL
+        launch_prec<false>(b, nb, g2p, p2g, gV, gO, la);
     return hipGetLastError();
 }
 
@@ -568,6 +768,7 @@ void icempm_batch_destroy(icempm_batch *b) {
     dfree(b->pos); dfree(b->rest); dfree(b->crushed); dfree(b->gridA); dfree(b->gridB);
     dfree(b->cases); dfree(b->chunkStart); dfree(b->chunkCase); dfree(b->nodeBlock);
     dfree(b->err); dfree(b->tTab); dfree(b->hist);
+    dfree(b->mlpW); dfree(b->mlpP); dfree(b->hx); dfree(b->ht); dfree(b->hcount); dfree(b->stats);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -672,6 +873,8 @@ int icempm_batch_finalize(icempm_batch *b) {
     BCHK(b, hipMalloc(&b->chunkCase, ccase.size() * sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->nodeBlock, nblk.size() * sizeof(int2)), "batch finalize");
     BCHK(b, hipMalloc(&b->err, sizeof(int)), "batch finalize");
+    BCHK(b, hipMalloc(&b->hcount, sizeof(unsigned long long)), "batch finalize");
+    BCHK(b, hipMalloc(&b->stats, 3 * sizeof(unsigned long long)), "batch finalize");
     // host staging in the device layout
     std::vector<double> hpos(2 * b->pitch, 0.0);
     std::vector<unsigned char> hrest(11 * b->pitch * vs, 0), hcr(b->pitch, 0);
@@ -704,6 +907,9 @@ int icempm_batch_finalize(icempm_batch *b) {
     BCHK(b, hipMemcpy(b->nodeBlock, nblk.data(), nblk.size() * sizeof(int2), hipMemcpyHostToDevice), "batch upload");
     int zero = 0;
     BCHK(b, hipMemcpy(b->err, &zero, sizeof(int), hipMemcpyHostToDevice), "batch upload");
+    const unsigned long long z3[3] = {0, 0, 0};
+    BCHK(b, hipMemcpy(b->hcount, z3, sizeof(unsigned long long), hipMemcpyHostToDevice), "batch upload");
+    BCHK(b, hipMemcpy(b->stats, z3, sizeof(z3), hipMemcpyHostToDevice), "batch upload");
     b->finalized = true;
     return ICEMPM_OK;
 }
@@ -825,6 +1031,85 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
 int icempm_batch_error_flag(icempm_batch *b, int *flag) {
     if (!b || !flag) return ICEMPM_E_ARG;
     BCHK(b, hipMemcpy(flag, b->err, sizeof(int), hipMemcpyDeviceToHost), "error flag");
+    return ICEMPM_OK;
+}
+
+int icempm_batch_set_return_map(icempm_batch *b, int mode, const float *params, const float *norm,
+                                double tol, int harvest_capacity) {
+    if (!b) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "set_return_map: batch not finalized");
+    if (mode != kMapExact && mode != kMapHarvest && mode != kMapLearned)
+        return bfail(b, ICEMPM_E_ARG, "set_return_map: unknown mode");
+    if (mode == kMapHarvest && harvest_capacity <= 0)
+        return bfail(b, ICEMPM_E_ARG, "set_return_map: harvest needs a positive capacity");
+    if (mode == kMapLearned && (!params || !norm))
+        return bfail(b, ICEMPM_E_ARG, "set_return_map: the learned map needs parameters and a normalisation");
+    if (mode == kMapLearned && !(tol == tol))
+        return bfail(b, ICEMPM_E_ARG, "set_return_map: tolerance is NaN");
+    BCHK(b, hipStreamSynchronize(b->stream), "set_return_map");
+    // a captured loop holds the previous kernel arguments
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->graphSteps = 0;
+    }
+    if (mode == kMapHarvest) {
+        if (harvest_capacity != b->hcap) {
+            dfree(b->hx);
+            dfree(b->ht);
+            b->hcap = 0;
+            BCHK(b, hipMalloc(&b->hx, (size_t)harvest_capacity * 5 * sizeof(float)), "set_return_map");
+            BCHK(b, hipMalloc(&b->ht, (size_t)harvest_capacity * 3 * sizeof(float)), "set_return_map");
+            b->hcap = harvest_capacity;
+        }
+        const unsigned long long zero = 0;
+        BCHK(b, hipMemcpy(b->hcount, &zero, sizeof(zero), hipMemcpyHostToDevice), "set_return_map");
+    }
+    if (mode == kMapLearned) {
+        const int np = mlpdev::kParams;
+        std::vector<unsigned short> wb(np);
+        for (int i = 0; i < np; ++i) wb[i] = mlpdev::to_bf16(params[i]);
+        if (!b->mlpW) BCHK(b, hipMalloc(&b->mlpW, np * sizeof(unsigned short)), "set_return_map");
+        if (!b->mlpP) BCHK(b, hipMalloc(&b->mlpP, np * sizeof(float)), "set_return_map");
+        BCHK(b, hipMemcpy(b->mlpW, wb.data(), np * sizeof(unsigned short), hipMemcpyHostToDevice), "set_return_map");
+        BCHK(b, hipMemcpy(b->mlpP, params, np * sizeof(float), hipMemcpyHostToDevice), "set_return_map");
+        std::memcpy(b->norm.in_mean, norm, 5 * sizeof(float));
+        std::memcpy(b->norm.in_scale, norm + 5, 5 * sizeof(float));
+        std::memcpy(b->norm.out_mean, norm + 10, 3 * sizeof(float));
+        std::memcpy(b->norm.out_scale, norm + 13, 3 * sizeof(float));
+        for (int q = 0; q < 5; ++q)
+            if (!(b->norm.in_scale[q] > 0)) return bfail(b, ICEMPM_E_ARG, "set_return_map: input scale must be > 0");
+        b->tol = tol;
+    }
+    b->rmap = mode;
+    return ICEMPM_OK;
+}
+
+int icempm_batch_read_harvest(icempm_batch *b, float *X, float *T, int cap, unsigned long long *total) {
+    if (!b || !total || cap < 0 || (cap > 0 && (!X || !T))) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "read_harvest: batch not finalized");
+    BCHK(b, hipStreamSynchronize(b->stream), "read_harvest");
+    unsigned long long n = 0;
+    BCHK(b, hipMemcpy(&n, b->hcount, sizeof(n), hipMemcpyDeviceToHost), "read_harvest");
+    *total = n;
+    const int m = (int)std::min<unsigned long long>(std::min<unsigned long long>(n, (unsigned long long)b->hcap),
+                                                    (unsigned long long)cap);
+    if (m > 0) {
+        BCHK(b, hipMemcpy(X, b->hx, (size_t)m * 5 * sizeof(float), hipMemcpyDeviceToHost), "read_harvest");
+        BCHK(b, hipMemcpy(T, b->ht, (size_t)m * 3 * sizeof(float), hipMemcpyDeviceToHost), "read_harvest");
+    }
+    return ICEMPM_OK;
+}
+
+int icempm_batch_map_stats(icempm_batch *b, unsigned long long *out, int reset) {
+    if (!b || !out) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "map_stats: batch not finalized");
+    BCHK(b, hipStreamSynchronize(b->stream), "map_stats");
+    BCHK(b, hipMemcpy(out, b->stats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "map_stats");
+    if (reset) {
+        const unsigned long long z3[3] = {0, 0, 0};
+        BCHK(b, hipMemcpy(b->stats, z3, sizeof(z3), hipMemcpyHostToDevice), "map_stats");
+    }
     return ICEMPM_OK;
 }
 

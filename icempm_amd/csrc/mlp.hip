@@ -23,24 +23,12 @@
 
 #include "../../include/icempm.h"
 #include "internal.h"
+#include "mlp_device.h"
 
 namespace icempm {
 namespace mlp {
 
-constexpr int kIn = 16, kH = 64, kOut = 16;
-constexpr int oW1 = 0, oB1 = 1024, oW2 = 1088, oB2 = 5184, oW3 = 5248, oB3 = 6272, kParams = 6288;
-constexpr int kRealIn = 5, kRealOut = 3;
-
-typedef short s4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-// float -> bf16 bits, round to nearest even (NaN kept quiet)
-static __device__ __host__ __forceinline__ unsigned short to_bf16(float f) {
-    unsigned int u = __builtin_bit_cast(unsigned int, f);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (unsigned short)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
+using namespace mlpdev;
 
 // One 16x16 output tile of A(16 x K) * B(K x 16) accumulated into acc.
 // a(m, k) and b(k, n) return the operands as floats (rounded to BF16 here).
@@ -124,35 +112,28 @@ static __device__ __forceinline__ void forward16(WaveScratch &s, const PW &P) {
     (void)g;
 }
 
-struct GlobalParams {
-    const float *p;
-    __device__ float operator[](int i) const { return p[i]; }
-};
-
-// Inference: rows of 16 normalised inputs (only the first 5 used) -> 3
-// normalised outputs (stride 3).
-__global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ params,
+// Inference: rows of normalised inputs (n x 5) -> normalised outputs (n x 3),
+// by the same wavefront routine as the learned return map of the batch pass.
+__global__ __launch_bounds__(256) void k_mlp_infer(const unsigned short *__restrict__ wbf,
+                                                   const float *__restrict__ params,
                                                    const float *__restrict__ X, int n,
                                                    float *__restrict__ Y) {
-    __shared__ WaveScratch ws[4];
+    __shared__ InferScratch ws[4];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    WaveScratch &s = ws[w];
-    const GlobalParams P{params};
+    InferScratch &s = ws[w];
     const int tiles = (n + 15) / 16;
     for (int t = blockIdx.x * 4 + w; t < tiles; t += gridDim.x * 4) {
         for (int e = l; e < 16 * kIn; e += 64) {
             const int m = e / kIn, k = e % kIn, row = t * 16 + m;
-            s.x[m][k] = (row < n && k < kRealIn) ? X[(size_t)row * kRealIn + k] : 0.f;
+            s.x[m][k] = to_bf16((row < n && k < kRealIn) ? X[(size_t)row * kRealIn + k] : 0.f);
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        forward16(s, P);
+        wave_fence();
+        infer16(s, wbf, params);
         for (int e = l; e < 16 * kRealOut; e += 64) {
             const int m = e / kRealOut, k = e % kRealOut, row = t * 16 + m;
             if (row < n) Y[(size_t)row * kRealOut + k] = s.y[m][k];
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        wave_fence();
     }
 }
 
@@ -294,10 +275,11 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ para
 
 // grad = sum over the partial vectors; then one Adam step on the FP32
 // master parameters.  Parameters that are padding stay zero.
-__global__ void k_mlp_adam(float *__restrict__ params, float *__restrict__ mom1,
-                           float *__restrict__ mom2, const float *__restrict__ part, int nparts,
-                           float *__restrict__ grad, float lr, float b1, float b2, float eps,
-                           float c1, float c2) {
+// The BF16 copy used by inference is refreshed from the result.
+__global__ void k_mlp_adam(float *__restrict__ params, unsigned short *__restrict__ wbf,
+                           float *__restrict__ mom1, float *__restrict__ mom2,
+                           const float *__restrict__ part, int nparts, float *__restrict__ grad,
+                           float lr, float b1, float b2, float eps, float c1, float c2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kParams) return;
     float gsum = 0;
@@ -310,14 +292,17 @@ __global__ void k_mlp_adam(float *__restrict__ params, float *__restrict__ mom1,
     else if (i >= oB3) pad = (i - oB3) >= kRealOut;
     if (pad) {
         params[i] = 0;
+        wbf[i] = 0;
         return;
     }
-    if (lr == 0) return;
-    const float m = b1 * mom1[i] + (1 - b1) * gsum;
-    const float v = b2 * mom2[i] + (1 - b2) * gsum * gsum;
-    mom1[i] = m;
-    mom2[i] = v;
-    params[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+    if (lr != 0) {
+        const float m = b1 * mom1[i] + (1 - b1) * gsum;
+        const float v = b2 * mom2[i] + (1 - b2) * gsum * gsum;
+        mom1[i] = m;
+        mom2[i] = v;
+        params[i] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+    }
+    wbf[i] = to_bf16(params[i]);
 }
 
 }  // namespace mlp
@@ -328,6 +313,7 @@ using namespace icempm::mlp;
 struct icempm_mlp {
     hipStream_t stream = nullptr;
     float *params = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr, *part = nullptr;
+    unsigned short *wbf = nullptr;   // params as BF16 bits
     float *loss = nullptr;
     int nparts = 0;
     int t = 0;   // Adam step
@@ -364,7 +350,11 @@ int icempm_mlp_create(icempm_mlp **out, const float *init) {
     if (e == hipSuccess) e = hipMalloc(&m->grad, kParams * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&m->part, (size_t)m->nparts * kParams * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&m->loss, sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&m->wbf, kParams * sizeof(unsigned short));
     if (e == hipSuccess) e = hipMemcpy(m->params, init, kParams * sizeof(float), hipMemcpyHostToDevice);
+    std::vector<unsigned short> wb(kParams);
+    for (int i = 0; i < kParams; ++i) wb[i] = to_bf16(init[i]);
+    if (e == hipSuccess) e = hipMemcpy(m->wbf, wb.data(), kParams * sizeof(unsigned short), hipMemcpyHostToDevice);
     std::vector<float> z(kParams, 0.f);
     if (e == hipSuccess) e = hipMemcpy(m->m1, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->m2, z.data(), kParams * sizeof(float), hipMemcpyHostToDevice);
@@ -381,6 +371,7 @@ void icempm_mlp_destroy(icempm_mlp *m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (float *p : {m->params, m->m1, m->m2, m->grad, m->part, m->loss})
         if (p) (void)hipFree(p);
+    if (m->wbf) (void)hipFree(m->wbf);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -409,8 +400,8 @@ int icempm_mlp_step(icempm_mlp *m, const float *X, const float *T, int r0, int B
     if (lr > 0) m->t += 1;
     const float c1 = 1 - std::pow(b1, (float)(m->t > 0 ? m->t : 1));
     const float c2 = 1 - std::pow(b2, (float)(m->t > 0 ? m->t : 1));
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, m->params, m->m1,
-                       m->m2, m->part, m->nparts, m->grad, lr, b1, b2, 1e-8f, c1, c2);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, m->params, m->wbf,
+                       m->m1, m->m2, m->part, m->nparts, m->grad, lr, b1, b2, 1e-8f, c1, c2);
     MCHK(m, hipGetLastError(), "mlp adam");
     if (grad_out)
         MCHK(m, hipMemcpyAsync(grad_out, m->grad, kParams * sizeof(float), hipMemcpyDeviceToHost, m->stream),
@@ -429,7 +420,7 @@ int icempm_mlp_infer(icempm_mlp *m, const float *X, int n, float *Y) {
     const int tiles = (n + 15) / 16;
     int blocks = (tiles + 3) / 4;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, m->stream, m->params, X, n, Y);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, m->stream, m->wbf, m->params, X, n, Y);
     MCHK(m, hipGetLastError(), "mlp infer");
     MCHK(m, hipStreamSynchronize(m->stream), "mlp infer");
     return ICEMPM_OK;

@@ -1,0 +1,104 @@
+// mlp_device.h -- device pieces of the learned return map shared by the
+// training kernels (mlp.hip) and the batch particle pass (batch.hip).
+//
+// Network: x (5, padded to 16) -> tanh(W1 x + b1) (64) -> tanh(W2 h + b2) (64)
+// -> W3 h + b3 (3, padded to 16).  Parameters as one flat vector:
+//   W1[64][16] @0, b1[64] @1024, W2[64][64] @1088, b2[64] @5184,
+//   W3[16][64] @5248, b3[16] @6272; 6288 values; W[out][in].
+// Products run on v_mfma_f32_16x16x16_bf16 with FP32 accumulation.  Lane
+// layout (wavefront of 64, g = lane/16, c = lane%16): lane holds
+// A[c][4g..4g+3], B[4g..4g+3][c] and receives D[4g+i][c], i < 4.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace icempm {
+namespace mlpdev {
+
+constexpr int kIn = 16, kH = 64, kOut = 16;
+constexpr int oW1 = 0, oB1 = 1024, oW2 = 1088, oB2 = 5184, oW3 = 5248, oB3 = 6272, kParams = 6288;
+constexpr int kRealIn = 5, kRealOut = 3;
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bits, round to nearest even (NaN kept quiet)
+static __device__ __host__ __forceinline__ unsigned short to_bf16(float f) {
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (unsigned short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+static __device__ __host__ __forceinline__ float from_bf16(unsigned short b) {
+    return __builtin_bit_cast(float, (unsigned int)b << 16);
+}
+
+static __device__ __forceinline__ void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One 16x16 output tile of A(16 x K) * B(K x 16) accumulated into acc;
+// a(m, k), b(k, n) return BF16 bits.
+template <class FA, class FB>
+static __device__ __forceinline__ f4 tile_bits(FA a, FB b, int K, f4 acc) {
+    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        s4 av, bv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            av[j] = (short)a(r, k0 + 4 * g + j);
+            bv[j] = (short)b(k0 + 4 * g + j, r);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Inference scratch of one wavefront: 16 rows, activations as BF16 bits.
+// Rows are padded by one dword so that the 16 rows of an operand fetch fall
+// in different LDS banks.
+constexpr int kInLd = kIn + 2, kHLd = kH + 2;
+struct InferScratch {
+    unsigned short x[16][kInLd];
+    unsigned short h1[16][kHLd];
+    unsigned short h2[16][kHLd];
+    float y[16][kOut + 1];
+};
+
+// Forward of the 16 rows in s.x.  Wbf: parameters as BF16 bits (weights),
+// Pf: the same parameters as FP32 (biases are added in FP32).
+static __device__ __forceinline__ void infer16(InferScratch &s, const unsigned short *__restrict__ Wbf,
+                                               const float *__restrict__ Pf) {
+    const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+    for (int nt = 0; nt < kH / 16; ++nt) {
+        f4 acc = {0, 0, 0, 0};
+        acc = tile_bits([&](int m, int k) { return s.x[m][k]; },
+                        [&](int k, int n) { return Wbf[oW1 + (nt * 16 + n) * kIn + k]; }, kIn, acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s.h1[4 * g + i][nt * 16 + c] = to_bf16(tanhf(acc[i] + Pf[oB1 + nt * 16 + c]));
+    }
+    wave_fence();
+    for (int nt = 0; nt < kH / 16; ++nt) {
+        f4 acc = {0, 0, 0, 0};
+        acc = tile_bits([&](int m, int k) { return s.h1[m][k]; },
+                        [&](int k, int n) { return Wbf[oW2 + (nt * 16 + n) * kH + k]; }, kH, acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s.h2[4 * g + i][nt * 16 + c] = to_bf16(tanhf(acc[i] + Pf[oB2 + nt * 16 + c]));
+    }
+    wave_fence();
+    f4 acc = {0, 0, 0, 0};
+    acc = tile_bits([&](int m, int k) { return s.h2[m][k]; },
+                    [&](int k, int n) { return Wbf[oW3 + n * kH + k]; }, kH, acc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s.y[4 * g + i][c] = acc[i] + Pf[oB3 + c];
+    wave_fence();
+}
+
+// Normalisation of the map's inputs and outputs (feature = (raw - mean) / scale).
+struct Norm {
+    float in_mean[5], in_scale[5], out_mean[3], out_scale[3];
+};
+
+}  // namespace mlpdev
+}  // namespace icempm

@@ -33,6 +33,10 @@ struct KParams {
     int nPts, gridX, gridY, gridTotal, nbins;
     double dt, vol, mass, h, h_inv, Dinv, mu, kappa, gravity, indRsq, indV, vmax;
     double ics, its, iss, grainVar, tanPhi, dpThr;
+    // with GrainVariability == 0 every grain has the constants of grain 0:
+    // they travel here and the table is not read
+    int uniformGrain;
+    double g0[5];
 };
 
 // ----------------------------------------------------------------------------
@@ -158,15 +162,22 @@ enum { kGPmax = 0, kGMsq = 1, kGBetaPmax = 2, kGOnePlus2Beta = 3, kGSlope = 4 };
 
 // the same test as fails(), reading the per-grain constants from the table
 static __device__ __forceinline__ bool fails_tab(const KParams &k, short grain, double p, double q) {
-    const double *g = k.grainTab + ((int)grain + 32768) * kGrainFields;
+    double pmax, msq, betaPmax, onePlus2Beta, slope;
+    if (k.uniformGrain) {
+        pmax = k.g0[kGPmax]; msq = k.g0[kGMsq]; betaPmax = k.g0[kGBetaPmax];
+        onePlus2Beta = k.g0[kGOnePlus2Beta]; slope = k.g0[kGSlope];
+    } else {
+        const double *g = k.grainTab + ((int)grain + 32768) * kGrainFields;
+        pmax = g[kGPmax]; msq = g[kGMsq]; betaPmax = g[kGBetaPmax];
+        onePlus2Beta = g[kGOnePlus2Beta]; slope = g[kGSlope];
+    }
     const double pmin2 = -3e6;
     if (p < 0) {
         if (p < pmin2) return true;
-        const double q_limit = g[kGSlope] * (p - pmin2);
+        const double q_limit = slope * (p - pmin2);
         return q > q_limit;
     }
-    const double pmax = g[kGPmax];
-    const double y = g[kGOnePlus2Beta] * q * q + g[kGMsq] * (p + g[kGBetaPmax]) * (p - pmax);
+    const double y = onePlus2Beta * q * q + msq * (p + betaPmax) * (p - pmax);
     return y > 0;
 }
 

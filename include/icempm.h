@@ -221,6 +221,33 @@ int icempm_batch_force_history(icempm_batch *b, int c, double *out, int n);
 /* the state of case c in the reference SoA layout (P, Q and grain zero) */
 int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch);
 int icempm_batch_error_flag(icempm_batch *b, int *flag);
+/* Return map of crushed points (gpu_implementation5.cu:48-94 is the exact
+ * one).  ICEMPM_MAP_EXACT: the exact FP64 map.  ICEMPM_MAP_HARVEST: the
+ * exact map, and every crushed point with positive singular values also
+ * records one (features, targets) pair, up to harvest_capacity pairs.
+ * ICEMPM_MAP_LEARNED: the branch of the map is decided exactly (tension,
+ * shear projection, or inside the surface, where the map is the identity);
+ * for the two projecting branches the MLP below (params:
+ * icempm_mlp_num_params() floats; norm: 16 floats = input mean[5], input
+ * scale[5], output mean[3], output scale[3]) proposes the projected state
+ * and the exact map replaces it wherever the yield residual (stress error
+ * relative to the case's strengths) exceeds tol.
+ * Pairs are harvested from the projecting branches only.  Features: log s0,
+ * log s1, log Jp_inv of the trial state, 0 (no hardening variable), 1 in
+ * the tension branch else 0.  Targets: the change of log s0, log s1 and
+ * log Jp_inv made by the map. */
+enum { ICEMPM_MAP_EXACT = 0, ICEMPM_MAP_HARVEST = 1, ICEMPM_MAP_LEARNED = 2 };
+int icempm_batch_set_return_map(icempm_batch *b, int mode, const float *params, const float *norm,
+                                double tol, int harvest_capacity);
+/* pairs recorded since harvest mode was set: *total = pairs offered (may
+ * exceed the capacity); copies min(total, capacity, cap) rows of X (n x 5)
+ * and T (n x 3) */
+int icempm_batch_read_harvest(icempm_batch *b, float *X, float *T, int cap,
+                              unsigned long long *total);
+/* out[0] points accepted from the MLP, out[1] points that fell back to the
+ * exact map, out[2] crushed points inside the surface (no projection), since
+ * the last reset; reset != 0 zeroes the counters after reading */
+int icempm_batch_map_stats(icempm_batch *b, unsigned long long *out, int reset);
 
 /* -- the learned return map (no counterpart) -----------------------------
  * A 5 -> 64 -> 64 -> 3 MLP (tanh hidden layers) with BF16 weights and

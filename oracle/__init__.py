@@ -115,27 +115,43 @@ def dp_return(prms, F, Jp_inv: float) -> tuple[np.ndarray, float]:
 
 
 # -- substep pieces on SoA buffers ---------------------------------------------
+def _check_buffers(prms, pts=None, grid=None, bins=None) -> None:
+    """The C routines index by prms' sizes; refuse buffers smaller than that."""
+    if grid is not None and (grid.shape != (3, prms.nGridPitch) or prms.nGridPitch < prms.GridTotal):
+        raise ValueError("grid must be (3, nGridPitch) with nGridPitch >= GridTotal")
+    if pts is not None and (pts.shape != (16, prms.nPtsPitch) or prms.nPtsPitch < prms.nPts):
+        raise ValueError("points must be (16, nPtsPitch) with nPtsPitch >= nPts")
+    if bins is not None and bins.size < 2 * prms.n_indenter_subdivisions:
+        raise ValueError("bins must hold 2 * n_indenter_subdivisions doubles")
+
+
 def new_grid(prms) -> np.ndarray:
+    if prms.nGridPitch < prms.GridTotal:
+        raise ValueError("set prms.nGridPitch (>= GridTotal) first")
     return np.zeros((3, prms.nGridPitch), dtype=np.float64)
 
 
 def p2g(prms, pts: np.ndarray, grid: np.ndarray) -> int:
+    _check_buffers(prms, pts, grid)
     return lib().oracle_p2g(ctypes.byref(prms), _dp(pts), _dp(grid))
 
 
 def update_nodes(prms, grid: np.ndarray, bins: np.ndarray, ix: float, iy: float) -> np.ndarray:
+    _check_buffers(prms, None, grid, bins)
     fs = np.zeros(2)
     lib().oracle_update_nodes(ctypes.byref(prms), _dp(grid), _dp(bins), ix, iy, _dp(fs))
     return fs
 
 
 def g2p(prms, pts: np.ndarray, grid: np.ndarray, record: bool) -> int:
+    _check_buffers(prms, pts, grid)
     return lib().oracle_g2p(ctypes.byref(prms), _dp(pts), _dp(grid), 1 if record else 0)
 
 
 def run(prms, pts: np.ndarray, grid: np.ndarray, bins: np.ndarray, nsub: int,
         nthreads: int = 1) -> tuple[np.ndarray, int]:
     """nsub substeps with Model::Step semantics (prms is advanced)."""
+    _check_buffers(prms, pts, grid, bins)
     hist = np.zeros((max(nsub, 1), 2), dtype=np.float64)
     err = lib().oracle_run(ctypes.byref(prms), _dp(pts), _dp(grid), _dp(bins), nsub, _dp(hist),
                            nthreads)
