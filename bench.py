@@ -1,16 +1,24 @@
 """Throughput of the MLS-MPM substep on one MI355X.
 
-Workload: the plain FP64 indentation case at scale -- an ice block 5 m x 2 m
-on a jittered lattice with 4 points per cell at h = 0.005 m (1,600,000
-points), grid 1200 x 500, rigid disc r = 0.3 m at 0.5 m/s, dt = 1e-6 s.
-A "step" is one substep (P2G -> grid update -> G2P with the NACC /
-Drucker-Prager return map) over all points; inputs are resident in HBM
-before the timed region starts.
+Default workload (--workload batch-learned): 32 independent indentation
+cases of uneven size (point counts log-uniform in [100k, 500k], ~8M points
+in total; Young's modulus, tensile strength and disc speed drawn per case)
+advanced together by the batched engine with FP32 point storage, FP64
+arithmetic and the learned return map (BF16 MLP on MFMA, exact FP64 map as
+fallback above a yield residual of 1e-4).  A "step" is one substep of every
+case.  Untimed set-up: pairs are harvested from an exact FP64 batched run of
+the same cases (which is also the force reference), the MLP is trained on the
+GPU, and the plain engines are timed on the same cases for comparison: the
+single-case engine case by case (the plain MPM), the batched engine with FP64
+and FP32 storage and the exact map.
+
+--workload single: the plain FP64 single-case substep at scale (ice block
+5 m x 2 m, 1,600,000 points, grid 1200 x 500).
 
 Prints one JSON line: particle-updates/s, the roofline of the dominant
 kernel (the fused G2P+P2G pass, timed with HIP events on its stream), and the
 CPU baseline (the C++ restatement, OpenMP, on this host's cores, on a
-bounded prefix of the same workload).
+bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -35,16 +43,23 @@ FUSED_BYTES_PER_NODE = 40
 # grid: P2G out 24 + update in 24 / out 16 + G2P in 16 = 80 B per node)
 SUBSTEP_BYTES_PER_POINT = 212
 SUBSTEP_BYTES_PER_NODE = 80
+# the batched pass with FP32 storage (positions FP64):
+#   per point: read pos 16 + Fe-I 16 + Jp 4 + crushed 1 = 37 B,
+#              write pos 16 + vel 8 + Fe-I 16 + Bp 16 = 56 B (Jp, crushed
+#              only when they change)
+BATCH_F32_BYTES_PER_POINT = 93
+BATCH_F64_BYTES_PER_POINT = 16 + 32 + 8 + 1 + 16 + 16 + 32 + 32
+METRIC = "particle-updates/s on 1 GPU: batched+learned vs plain MPM at ≤1% force err"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(case_name: str):
+def traffic_from_profiles(name: str):
     """HBM bytes per launch of the fused kernel, from the committed rocprofv3
-    counter summary (profiles/fused_g2p_p2g_<case>_counters.txt), if any."""
-    path = os.path.join(ROOT, "profiles", f"fused_g2p_p2g_{case_name}_counters.txt")
+    counter summary (profiles/<name>_counters.txt), if any."""
+    path = os.path.join(ROOT, "profiles", f"{name}_counters.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -73,7 +88,7 @@ def cpu_baseline(case, pts0, prms0, target_s: float, threads: int):
     t0 = time.perf_counter()
     oracle.run(p, pts, grid, bins, 1, nthreads=threads)
     one = time.perf_counter() - t0
-    nsub = max(1, min(200, int(target_s / max(one, 1e-6))))
+    nsub = max(1, min(2000, int(target_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     _, err = oracle.run(p, pts, grid, bins, nsub, nthreads=threads)
     el = time.perf_counter() - t0
@@ -87,33 +102,22 @@ def cpu_baseline(case, pts0, prms0, target_s: float, threads: int):
     }
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--case", default="indent-1.6M")
-    ap.add_argument("--mode", choices=["fast", "ordered"], default="fast")
-    ap.add_argument("--pass-kind", type=int, default=None,
-                    help="particle-pass kernel variant (default: the library's)")
-    ap.add_argument("--timing-stride", type=int, default=8,
-                    help="bracket every k-th launch of the dominant kernel with HIP events")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
+def fail_gpus() -> int:
+    print("bench.py: this build measures one GPU only; runs on more than one GPU are not "
+          "supported", file=sys.stderr)
+    return 2
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only (the substep of one case on one "
-              "MI355X); runs on more than one GPU are not supported", file=sys.stderr)
-        return 2
 
+def run_single(args) -> dict:
+    """The plain FP64 single-case engine on one case (default indent-1.6M)."""
     import numpy as np
 
     from icempm_amd import CASES, MODE_FAST, MODE_ORDERED, Model
 
     case = CASES[args.case]
+    steps, warmup = args.steps, args.warmup
     log(f"bench: {case.name} n_points={case.n_points} grid={case.grid_x}x{case.grid_y} "
-        f"mode={args.mode} warmup={args.warmup} steps={args.steps}")
+        f"mode={args.mode} warmup={warmup} steps={steps}")
     m = Model(MODE_FAST if args.mode == "fast" else MODE_ORDERED)
     if args.pass_kind is not None:
         m.set_pass_kind(args.pass_kind)
@@ -123,27 +127,26 @@ def main() -> int:
     pts0 = m.points() if not args.no_cpu_baseline else None
     prms0 = m.prms.copy()
 
-    m.run(args.warmup)
+    m.run(warmup)
     m.synchronize()
     log("bench: warm-up done")
 
     m.kernel_timing(args.timing_stride)
     m.synchronize()
     t0 = time.perf_counter()
-    hist = m.run(args.steps)          # ends with a device->host copy (synchronising)
+    hist = m.run(steps)          # ends with a device->host copy (synchronising)
     m.synchronize()
     el = time.perf_counter() - t0
     ktot, klaunch = m.kernel_timing_read()
     m.kernel_timing(0)
     err = m.error_flag()
     if err or not np.isfinite(hist).all():
-        log("bench: run flagged an out-of-grid point or a non-finite force")
-        return 1
+        raise RuntimeError("bench: run flagged an out-of-grid point or a non-finite force")
 
     n = case.n_points
     G = case.grid_x * case.grid_y
-    value = n * args.steps / el
-    ms_per_step = el / args.steps * 1e3
+    value = n * steps / el
+    ms_per_step = el / steps * 1e3
     k_avg_s = (ktot / max(klaunch, 1)) / 1e3
     k_bytes = n * FUSED_BYTES_PER_POINT + G * FUSED_BYTES_PER_NODE
     achieved = k_bytes / k_avg_s / 1e9 if klaunch else None
@@ -153,6 +156,7 @@ def main() -> int:
 
     copy_gbs = m.copy_bandwidth(1 << 30, 20)
     log(f"bench: device copy kernel {copy_gbs:.0f} GB/s (read + write)")
+    m.close()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -160,13 +164,13 @@ def main() -> int:
         log(f"bench: CPU baseline ({threads} threads, ~{args.cpu_seconds:.0f} s)")
         cpu = cpu_baseline(case, pts0, prms0, args.cpu_seconds, threads)
 
-    out = {
-        "metric": "particle-updates/s on 1 GPU: batched+learned vs plain MPM at ≤1% force err",
+    return {
+        "metric": METRIC,
         "value": value,
         "unit": "particle-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -184,22 +188,231 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_particles<G2P,P2G> (fused G2P + P2G)",
+            "kernel": "k_particles_wave<G2P,P2G> (fused G2P + P2G)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": traffic_from_profiles(case.name),
+            "traffic": traffic_from_profiles(f"fused_g2p_p2g_{case.name}"),
             "bytes_per_launch": k_bytes,
             "avg_launch_us": k_avg_s * 1e6,
             "timed_launches": klaunch,
             "copy_kernel_gbs": copy_gbs,
         },
-        "substep_effective_gbs": sub_bytes * args.steps / el / 1e9,
+        "substep_effective_gbs": sub_bytes * steps / el / 1e9,
         "cpu_baseline": cpu,
     }
+
+
+def rel_l2(a, b) -> float:
+    import numpy as np
+    n = float(np.linalg.norm(b))
+    d = float(np.linalg.norm(a - b))
+    return d / n if n > 0 else d
+
+
+def run_batch_learned(args) -> dict:
+    """32 uneven cases, batched, FP32 storage, learned return map; the plain
+    engines on the same cases for comparison."""
+    import numpy as np
+
+    from icempm_amd import MODE_FAST, Model
+    from icempm_amd._lib import BATCH_F32, BATCH_F64
+    from icempm_amd.batch import batch_cases, build_batch, ingest_cases
+    from icempm_amd.learned import train_return_map
+
+    steps, warmup = args.steps, args.warmup
+    cases = batch_cases(args.cases, args.n_min, args.n_max, args.seed)
+    t0 = time.perf_counter()
+    states = ingest_cases(cases)
+    n_total = sum(c.n_points for c in cases)
+    nodes = sum(c.grid_x * c.grid_y for c in cases)
+    log(f"bench: {len(cases)} cases, {n_total} points, {nodes} grid nodes, ingest "
+        f"{time.perf_counter() - t0:.1f} s; warmup={warmup} steps={steps}")
+
+    def timed(b, label):
+        """warm-up, then the timed steps (graph replay); the force histories of both runs"""
+        b.run(warmup)
+        h0 = [b.force_history(c, warmup) for c in range(len(cases))]
+        ms = b.run(steps)
+        h1 = [b.force_history(c, steps) for c in range(len(cases))]
+        if b.error_flag():
+            raise RuntimeError(f"bench: {label}: out-of-grid point")
+        hist = [np.concatenate([a, c]) for a, c in zip(h0, h1)]
+        if not all(np.isfinite(h).all() for h in hist):
+            raise RuntimeError(f"bench: {label}: non-finite force")
+        pups = n_total * steps / (ms / 1e3)
+        log(f"bench: {label}: {ms / steps * 1e3:.1f} us/substep, {pups:.3e} particle-updates/s")
+        return pups, ms, hist
+
+    # 1. exact FP64 batched run: force reference and harvested pairs
+    b = build_batch(cases, BATCH_F64, states)
+    b.set_harvest(args.harvest)
+    f64_harvest_pups, _, ref_hist = timed(b, "batched FP64 exact (harvest)")
+    X, T, offered = b.read_harvest(args.harvest)
+    b.close()
+    log(f"bench: harvested {offered} pairs, training on {X.shape[0]}")
+
+    # 2. the MLP, trained on the GPU
+    t0 = time.perf_counter()
+    params, norm, losses = train_return_map(X, T, steps=args.train_steps, batch=8192, lr=3e-3, seed=0)
+    train_s = time.perf_counter() - t0
+    log(f"bench: trained {args.train_steps} Adam steps in {train_s:.1f} s, loss {losses[-1][1]:.3e}")
+
+    # 3. plain batched engines
+    b = build_batch(cases, BATCH_F64, states)
+    f64_pups, _, _ = timed(b, "batched FP64 exact")
+    b.close()
+    b = build_batch(cases, BATCH_F32, states)
+    f32_pups, _, f32_hist = timed(b, "batched FP32 exact")
+    b.close()
+
+    # 4. the measured configuration: batched FP32 + learned map
+    b = build_batch(cases, BATCH_F32, states)
+    b.set_learned(params, norm, tol=args.tol)
+    value, ms, hist = timed(b, "batched FP32 learned")
+    acc, fb, ins = b.map_stats(reset=True)
+    errs = [rel_l2(h, r) for h, r in zip(hist, ref_hist)]
+    errs32 = [rel_l2(h, r) for h, r in zip(f32_hist, ref_hist)]
+    # the fused pass alone, timed with events on the batch stream
+    b.kernel_timing(args.timing_stride)
+    b.run(args.kernel_steps, use_graph=False)
+    ktot, klaunch = b.kernel_timing_read()
+    b.kernel_timing(0)
+    b.close()
+    k_avg_s = (ktot / max(klaunch, 1)) / 1e3
+    k_bytes = n_total * BATCH_F32_BYTES_PER_POINT + nodes * FUSED_BYTES_PER_NODE
+    achieved = k_bytes / k_avg_s / 1e9 if klaunch else None
+    log(f"bench: fused pass {k_avg_s * 1e6:.1f} us ({klaunch} timed launches), "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    # 5. the plain MPM: the single-case FP64 engine, case by case
+    seq_updates, seq_time = 0, 0.0
+    m = Model(MODE_FAST)
+    copy_gbs = None
+    try:
+        for c in cases:
+            m.load_case(c)
+            m.run(min(warmup, 20))
+            m.synchronize()
+            t0 = time.perf_counter()
+            m.run(args.plain_steps)
+            m.synchronize()
+            seq_time += time.perf_counter() - t0
+            seq_updates += c.n_points * args.plain_steps
+        copy_gbs = m.copy_bandwidth(1 << 30, 20)
+    finally:
+        m.close()
+    plain_pups = seq_updates / seq_time
+    log(f"bench: plain single-case FP64, case by case: {plain_pups:.3e} particle-updates/s; "
+        f"copy kernel {copy_gbs:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        mid = sorted(range(len(cases)), key=lambda i: cases[i].n_points)[len(cases) // 2]
+        log(f"bench: CPU baseline ({threads} threads, ~{args.cpu_seconds:.0f} s, case {mid})")
+        cpu = cpu_baseline(cases[mid], states[mid][1], states[mid][0], args.cpu_seconds, threads)
+
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered lattices, splitmix64; seeded case parameters)",
+        "config": {
+            "workload": f"batched + learned: {len(cases)} indentation cases, points log-uniform in "
+                        f"[{args.n_min}, {args.n_max}] ({n_total} total, 4 per cell, h=0.012 m), "
+                        f"E in [5e9, 10e9] Pa, tensile strength in [0.5e6, 2e6] Pa, disc speed in "
+                        f"[0.1, 1.0] m/s, dt=2e-6 s; FP32 point storage, FP64 arithmetic, learned "
+                        f"return map (BF16 MLP 5-64-64-3 on MFMA, exact FP64 fallback above a yield "
+                        f"residual of {args.tol:g})",
+            "cases": len(cases),
+            "points": n_total,
+            "grid_nodes": nodes,
+            "storage": "f32 (positions f64)",
+            "mlp": "bf16 weights/activations, f32 accumulation",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_batch_particles<F32,G2P,P2G,learned> (fused G2P + P2G, all cases)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": traffic_from_profiles("batch_particles_learned_32cases"),
+            "bytes_per_launch": k_bytes,
+            "avg_launch_us": k_avg_s * 1e6,
+            "timed_launches": klaunch,
+            "copy_kernel_gbs": copy_gbs,
+        },
+        "comparison": {
+            "plain_single_case_f64_pups": plain_pups,
+            "plain_steps_per_case": args.plain_steps,
+            "batched_f64_exact_pups": f64_pups,
+            "batched_f64_harvest_pups": f64_harvest_pups,
+            "batched_f32_exact_pups": f32_pups,
+            "batched_f32_learned_pups": value,
+            "speedup_vs_plain": value / plain_pups,
+            "speedup_vs_batched_f64": value / f64_pups,
+            "force_rel_l2_max_learned_vs_f64_exact": max(errs),
+            "force_rel_l2_max_f32_exact_vs_f64_exact": max(errs32),
+            "learned_accepted": acc,
+            "learned_fallback": fb,
+            "crushed_inside_surface": ins,
+            "fallback_rate": fb / max(acc + fb, 1),
+            "pairs_harvested": offered,
+            "pairs_trained": int(X.shape[0]),
+            "train_steps": args.train_steps,
+            "train_seconds": train_s,
+            "train_loss": losses[-1][1],
+        },
+        "cpu_baseline": cpu,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", choices=["batch-learned", "single"], default="batch-learned")
+    ap.add_argument("--case", default="indent-1.6M", help="single: the case")
+    ap.add_argument("--mode", choices=["fast", "ordered"], default="fast", help="single: the mode")
+    ap.add_argument("--pass-kind", type=int, default=None,
+                    help="single: particle-pass kernel variant (default: the library's)")
+    ap.add_argument("--cases", type=int, default=32, help="batch: number of cases")
+    ap.add_argument("--n-min", type=int, default=100_000)
+    ap.add_argument("--n-max", type=int, default=500_000)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--tol", type=float, default=1e-4, help="batch: yield-residual tolerance")
+    ap.add_argument("--harvest", type=int, default=2_000_000, help="batch: pairs kept for training")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--plain-steps", type=int, default=400,
+                    help="batch: substeps per case of the plain single-case comparison")
+    ap.add_argument("--kernel-steps", type=int, default=200,
+                    help="batch: stream-launched substeps for the fused-pass timing")
+    ap.add_argument("--timing-stride", type=int, default=4,
+                    help="bracket every k-th launch of the dominant kernel with HIP events")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        return fail_gpus()
+    try:
+        out = run_single(args) if args.workload == "single" else run_batch_learned(args)
+    except RuntimeError as e:
+        log(str(e))
+        return 1
     print(json.dumps(out), flush=True)
-    m.close()
     return 0
 
 

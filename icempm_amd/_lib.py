@@ -128,6 +128,7 @@ HIP_EXPORTS = [
     "icempm_batch_run", "icempm_batch_last_ms", "icempm_batch_num_points",
     "icempm_batch_force_history", "icempm_batch_read_points", "icempm_batch_error_flag",
     "icempm_batch_set_return_map", "icempm_batch_read_harvest", "icempm_batch_map_stats",
+    "icempm_batch_num_nodes", "icempm_batch_kernel_timing", "icempm_batch_kernel_timing_read",
     "icempm_mlp_num_params", "icempm_mlp_create", "icempm_mlp_destroy", "icempm_mlp_last_error",
     "icempm_mlp_get_params", "icempm_mlp_device_params", "icempm_mlp_step", "icempm_mlp_infer",
     "icempm_mfma_probe", "icempm_device_alloc", "icempm_device_free", "icempm_copy_to_device",
@@ -208,6 +209,9 @@ def hip():
     _sig(L, "icempm_batch_read_harvest", c_int, P, POINTER(c_float), POINTER(c_float), c_int,
          POINTER(c_uint64))
     _sig(L, "icempm_batch_map_stats", c_int, P, POINTER(c_uint64), c_int)
+    _sig(L, "icempm_batch_num_nodes", c_long, P)
+    _sig(L, "icempm_batch_kernel_timing", c_int, P, c_int)
+    _sig(L, "icempm_batch_kernel_timing_read", c_int, P, POINTER(c_double), POINTER(c_int))
     fp = POINTER(c_float)
     _sig(L, "icempm_mlp_num_params", c_int)
     _sig(L, "icempm_mlp_create", c_int, POINTER(c_void_p), fp)

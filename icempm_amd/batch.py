@@ -74,7 +74,12 @@ class Batch:
         """Ingest the case through the model (LoadRawPoints) and add its state."""
         model.load_case(case)
         soa = np.ascontiguousarray(model.points())
-        p = model.prms.copy()
+        return self.add_state(case, model.prms.copy(), soa)
+
+    def add_state(self, case: IndentationCase, prms: SimParams, soa: np.ndarray) -> int:
+        """Add a case from its parameters and point SoA (16 x pitch doubles)."""
+        soa = np.ascontiguousarray(soa, dtype=np.float64)
+        p = prms.copy()
         idx = self._H.icempm_batch_add_case(self._b, byref(p), ctypes.sizeof(SimParams),
                                             soa.ctypes.data_as(POINTER(c_double)), soa.shape[1])
         if idx < 0:
@@ -96,6 +101,19 @@ class Batch:
     @property
     def num_points(self) -> int:
         return int(self._H.icempm_batch_num_points(self._b))
+
+    @property
+    def num_nodes(self) -> int:
+        return int(self._H.icempm_batch_num_nodes(self._b))
+
+    def kernel_timing(self, stride: int) -> None:
+        """Time every stride-th fused particle pass of stream runs (0: off)."""
+        self._check(self._H.icempm_batch_kernel_timing(self._b, int(stride)))
+
+    def kernel_timing_read(self) -> tuple[float, int]:
+        ms, n = c_double(0), c_int(0)
+        self._check(self._H.icempm_batch_kernel_timing_read(self._b, byref(ms), byref(n)))
+        return ms.value, n.value
 
     def force_history(self, c: int, n: int) -> np.ndarray:
         out = np.zeros((n, 2))
@@ -154,13 +172,26 @@ class Batch:
         return int(out[0]), int(out[1]), int(out[2])
 
 
-def build_batch(cases: list[IndentationCase], precision: int = BATCH_F64) -> Batch:
-    b = Batch(precision)
+def ingest_cases(cases: list[IndentationCase]) -> list[tuple[SimParams, np.ndarray]]:
+    """Each case's parameters and point SoA after LoadRawPoints on the GPU."""
+    out = []
     m = Model(MODE_FAST)
     try:
         for c in cases:
-            b.add(m, c)
+            m.load_case(c)
+            out.append((m.prms.copy(), np.ascontiguousarray(m.points())))
     finally:
         m.close()
+    return out
+
+
+def build_batch(cases: list[IndentationCase], precision: int = BATCH_F64,
+                states: list[tuple[SimParams, np.ndarray]] | None = None) -> Batch:
+    """A finalized batch of the cases (ingested here unless states are given)."""
+    if states is None:
+        states = ingest_cases(cases)
+    b = Batch(precision)
+    for c, (p, soa) in zip(cases, states):
+        b.add_state(c, p, soa)
     b.finalize()
     return b

@@ -138,8 +138,7 @@ KParams make_kparams(const icempm_ctx *c) {
     k.grainVar = P.GrainVariability;
     k.tanPhi = P.DP_tan_phi;
     k.dpThr = P.DP_threshold_p;
-    k.uniformGrain = P.GrainVariability == 0 ? 1 : 0;
-    if (k.uniformGrain) grain_row(P, 0, k.g0);
+    grain_row(P, 0, k.g0);
     return k;
 }
 

@@ -40,7 +40,7 @@ struct CaseParams {
     int gx, gy, gridTotal, nPts, ptOffset, gridOffset, pad0, pad1;
 };
 
-constexpr int kWX = 48, kWS = 49, kWY = 4;
+constexpr int kWX = 48, kWS = 48, kWY = 4;
 
 template <bool F32>
 struct Rows {
@@ -95,6 +95,21 @@ struct LearnArgs {
     unsigned long long *stats;   // [accepted, fell back, inside the surface]
 };
 
+// A crushed point whose projection is deferred to the map kernel: its trial
+// elastic deformation (FP64) and Jp_inv, and where it lives.
+struct MapEntry {
+    int pt, cs;
+    double F[4];
+    double Jp;
+};
+
+// Queue of deferred points of one pass (learned map only).
+struct MapQueue {
+    MapEntry *q;
+    unsigned int *count;   // appended by the particle pass
+    int cap;
+};
+
 // The branch of the exact map (Wolper_Drucker_Prager, gpu_implementation5.cu:48-94)
 // a crushed trial state falls in, decided exactly: the tension branch, the
 // shear projection onto q = q_lim, or inside the surface (the map is the
@@ -143,11 +158,8 @@ static __device__ __forceinline__ double yield_residual(const KParams &k, const 
     return fmax(fabs(qt - qlim) / k.iss, k.kappa * fabs(dm) / k.ics);
 }
 
-// Bytes of shared memory per wavefront: the P2G window, which the learned
-// map's inference scratch reuses during G2P.
+// Bytes of the per-wavefront P2G window.
 constexpr int kWaveWinBytes = 3 * kWY * kWS * 8;
-constexpr int kWaveBytes = kWaveWinBytes > (int)sizeof(mlpdev::InferScratch) ? kWaveWinBytes
-                                                                             : (int)sizeof(mlpdev::InferScratch);
 
 // Fused particle pass over all cases (G2P of substep k and/or P2G of k+1).
 template <bool F32, bool G2P, bool P2G, int RMAP>
@@ -156,8 +168,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
                                                          const int *__restrict__ chunkCase,
                                                          int nChunks, const double *__restrict__ gridV,
                                                          double *__restrict__ gridOut, size_t gpitch,
-                                                         int *__restrict__ err, LearnArgs la) {
-    __shared__ __attribute__((aligned(16))) unsigned char smem[4][RMAP == kMapLearned ? kWaveBytes : kWaveWinBytes];
+                                                         int *__restrict__ err, LearnArgs la, MapQueue mq) {
+    __shared__ __attribute__((aligned(16))) double smem[4][kWaveWinBytes / 8];
     const int per = (nChunks + 7) >> 3;
     const int c = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (c >= nChunks) return;
@@ -179,6 +191,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
     double px = 0, py = 0, vx = 0, vy = 0;
     double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
     double B00 = 0, B01 = 0, B10 = 0, B11 = 0;
+    bool defer = false;   // learned map: projection and P2G done by the map kernel
 
     if constexpr (G2P) {
         double Jp = 1;
@@ -304,55 +317,34 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             }
         } else {
             // the branch is decided exactly; inside the surface the map is
-            // the identity, and the network proposes the projected state of
-            // the two other branches
+            // the identity; points that project are queued for the map
+            // kernel (network proposal, residual check, exact fallback) unless
+            // their state is inverted, which takes the exact map here
             double qlim = 0;
             const int br = crushedOk ? map_branch(k, r, Jp, qlim) : kBrInside;
             const bool proj = crushedOk && br != kBrInside;
-            const bool need = proj && r.svd.s1 > 0;
-            const unsigned long long mask = __ballot(need);
-            double pred[3] = {0, 0, 0};
-            if (mask) {
-                mlpdev::InferScratch &isc = *reinterpret_cast<mlpdev::InferScratch *>(smem[wv]);
-                const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-                const int cnt = __popcll(mask);
-                float xin[5] = {0, 0, 0, 0, 0};
-                if (need) {
-                    double f[5];
-                    map_features(r, Jp, br, f);
-                    for (int q = 0; q < 5; ++q)
-                        xin[q] = (float)((f[q] - la.norm.in_mean[q]) / la.norm.in_scale[q]);
-                }
-                for (int base = 0; base < cnt; base += 16) {
-                    for (int e = lane; e < 16 * mlpdev::kIn; e += 64) isc.x[e / mlpdev::kIn][e % mlpdev::kIn] = 0;
-                    mlpdev::wave_fence();
-                    if (need && rank >= base && rank < base + 16)
-                        for (int q = 0; q < 5; ++q) isc.x[rank - base][q] = mlpdev::to_bf16(xin[q]);
-                    mlpdev::wave_fence();
-                    mlpdev::infer16(isc, la.wbf, la.pf);
-                    if (need && rank >= base && rank < base + 16)
-                        for (int q = 0; q < 3; ++q)
-                            pred[q] = (double)isc.y[rank - base][q] * la.norm.out_scale[q] + la.norm.out_mean[q];
-                    mlpdev::wave_fence();
+            defer = proj && r.svd.s1 > 0;
+            if (proj && !defer) dp_return(k, r, F00, F01, F10, F11, Jp);
+            const unsigned long long dm = __ballot(defer);
+            if (dm) {
+                unsigned int base = 0;
+                if (lane == 0) base = atomicAdd(mq.count, (unsigned int)__popcll(dm));
+                base = __shfl(base, 0, 64);
+                const unsigned int slot = base + (unsigned int)__popcll(dm & ((1ull << lane) - 1ull));
+                if (defer) {
+                    if (slot < (unsigned int)mq.cap) {
+                        MapEntry &e = mq.q[slot];
+                        e.pt = pt;
+                        e.cs = chunkCase[c];
+                        e.F[0] = F00; e.F[1] = F01; e.F[2] = F10; e.F[3] = F11;
+                        e.Jp = Jp;
+                    } else {
+                        flag_error(k);
+                    }
                 }
             }
-            bool accepted = false;
-            if (need) {
-                const double s0 = exp(log(r.svd.s0) + pred[0]);
-                const double s1 = exp(log(r.svd.s1) + pred[1]);
-                // the shear branch leaves Jp_inv unchanged
-                const double Jpn = (br == kBrTension) ? Jp * exp(pred[2]) : Jp;
-                if (yield_residual(k, r, Jp, br, qlim, s0, s1, Jpn) <= la.tol) {
-                    usvt(r.svd, s0, s1, F00, F01, F10, F11);
-                    Jp = Jpn;
-                    accepted = true;
-                }
-            }
-            if (proj && !accepted) dp_return(k, r, F00, F01, F10, F11, Jp);
-            const unsigned long long acc = __ballot(accepted), fb = __ballot(proj && !accepted),
-                                     ins = __ballot(crushedOk && !proj);
-            if (lane == 0 && (acc | fb | ins)) {
-                if (acc) atomicAdd(&la.stats[0], (unsigned long long)__popcll(acc));
+            const unsigned long long fb = __ballot(proj && !defer), ins = __ballot(crushedOk && !proj);
+            if (lane == 0 && (fb | ins)) {
                 if (fb) atomicAdd(&la.stats[1], (unsigned long long)__popcll(fb));
                 if (ins) atomicAdd(&la.stats[2], (unsigned long long)__popcll(ins));
             }
@@ -362,15 +354,17 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             S.pos[pp + pt] = py;
             R[pt] = (V)vx;
             R[pp + pt] = (V)vy;
-            R[pp * 2 + pt] = (V)(F00 - fe0);
-            R[pp * 3 + pt] = (V)F01;
-            R[pp * 4 + pt] = (V)F10;
-            R[pp * 5 + pt] = (V)(F11 - fe0);
+            if (!defer) {
+                R[pp * 2 + pt] = (V)(F00 - fe0);
+                R[pp * 3 + pt] = (V)F01;
+                R[pp * 4 + pt] = (V)F10;
+                R[pp * 5 + pt] = (V)(F11 - fe0);
+            }
             R[pp * 6 + pt] = (V)B00;
             R[pp * 7 + pt] = (V)B01;
             R[pp * 8 + pt] = (V)B10;
             R[pp * 9 + pt] = (V)B11;
-            if (Jp != Jp0) R[pp * 10 + pt] = (V)Jp;
+            if (Jp != Jp0 && !defer) R[pp * 10 + pt] = (V)Jp;
             if (crushed != crushed0) S.crushed[pt] = crushed;
             if constexpr (F32) {
                 // continue the P2G from what was stored, as a separate pass would
@@ -400,16 +394,16 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
                 B11 = (double)R[pp * 9 + pt];
             }
         } else {
-            have = valid && !(px != px || py != py);
+            have = valid && !defer && !(px != px || py != py);
         }
         int bx = 0, by = 0;
         double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
         const bool ok = have && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
                         stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
-        if (valid && !ok) flag_error(k);
+        if (valid && !ok && !defer) flag_error(k);   // deferred points scatter in the map kernel
         const int ox = wmin(ok ? bx : INT_MAX);
         const int oy = wmin(ok ? by : INT_MAX);
-        double *W0 = reinterpret_cast<double *>(smem[wv]);
+        double *W0 = smem[wv];
         double *W1 = W0 + kWY * kWS, *W2 = W1 + kWY * kWS;
         for (int e = lane; e < kWS * kWY; e += 64) {
             W0[e] = 0;
@@ -503,6 +497,147 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
                     }
                 }
             }
+        }
+    }
+}
+
+// P2G of one point from its stored position, velocity and Bp and the given
+// Fe (rounded to the storage type first, as the pass continues from what it
+// stored), straight to the grid with global atomics.
+template <bool F32>
+static __device__ void scatter_point(const Rows<F32> &S, const CaseParams &cp, const KParams &k, int pt,
+                                     double F00, double F01, double F10, double F11,
+                                     double *__restrict__ gridOut, size_t gpitch) {
+    using V = typename Rows<F32>::V;
+    constexpr double fe0 = F32 ? 1.0 : 0.0;
+    const size_t pp = S.pitch;
+    const V *__restrict__ R = S.rest;
+    const double px = S.pos[pt], py = S.pos[pp + pt];
+    const double vx = (double)R[pt], vy = (double)R[pp + pt];
+    const double B00 = (double)R[pp * 6 + pt], B01 = (double)R[pp * 7 + pt];
+    const double B10 = (double)R[pp * 8 + pt], B11 = (double)R[pp * 9 + pt];
+    if constexpr (F32) {
+        F00 = fe0 + (double)(V)(F00 - fe0); F01 = (double)(V)F01;
+        F10 = (double)(V)F10; F11 = fe0 + (double)(V)(F11 - fe0);
+    }
+    int bx, by;
+    double fx0, fx1, wx0, wx1, wx2, wy0, wy1, wy2;
+    if (!(px == px && py == py) || !stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) ||
+        !stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2)) {
+        flag_error(k);
+        return;
+    }
+    {
+        double t00, t01, t11;
+        kirchhoff(F00, F01, F10, F11, cp.mu, cp.kappa, t00, t01, t11);
+        const double m = cp.mass, h = cp.h;
+        const double sc2 = cp.dt * cp.vol * cp.Dinv;
+        const double A00 = m * B00 - sc2 * t00, A01 = m * B01 - sc2 * t01;
+        const double A10 = m * B10 - sc2 * t01, A11 = m * B11 - sc2 * t11;
+        const double mv0 = vx * m, mv1 = vy * m;
+        double *__restrict__ gO = gridOut + cp.gridOffset;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                const int X = bx + a, Y = by + b;
+                if (X >= cp.gx || Y >= cp.gy) {
+                    flag_error(k);
+                    continue;
+                }
+                const double Wip = sel3(a, wx0, wx1, wx2) * sel3(b, wy0, wy1, wy2);
+                const double d0 = (a - fx0) * h, d1 = (b - fx1) * h;
+                const int node = X + Y * cp.gx;
+                unsafeAtomicAdd(&gO[node], Wip * m);
+                unsafeAtomicAdd(&gO[gpitch + node], Wip * (mv0 + (A00 * d0 + A01 * d1)));
+                unsafeAtomicAdd(&gO[2 * gpitch + node], Wip * (mv1 + (A10 * d0 + A11 * d1)));
+            }
+    }
+}
+
+// The learned return map of the points a particle pass deferred: the MLP
+// (one 16-row MFMA tile per wavefront and iteration, weights staged in
+// shared memory) proposes the projected state of
+// each queued point, the proposal is kept where its yield residual is <=
+// tol and the exact FP64 map runs otherwise; then Fe and Jp_inv are stored
+// and, when gridOut is given, the point's P2G contribution goes to the grid
+// with global atomics (the pass left these points out of its P2G).  Resets
+// the counter the next pass appends to.
+template <bool F32>
+__global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams *__restrict__ cases,
+                                                  const MapEntry *__restrict__ q,
+                                                  const unsigned int *__restrict__ qCount,
+                                                  unsigned int *__restrict__ qCountNext,
+                                                  double *__restrict__ gridOut, size_t gpitch,
+                                                  int *__restrict__ err, LearnArgs la) {
+    __shared__ mlpdev::InferScratch isc[4];
+    __shared__ unsigned short wsh[mlpdev::kParams];   // the weights, BF16, staged once per block
+    using V = typename Rows<F32>::V;
+    constexpr double fe0 = F32 ? 1.0 : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) qCountNext[0] = 0u;
+    const unsigned int n = qCount[0];
+    if (blockIdx.x * 64u >= n) return;   // no entry for this block (uniform)
+    for (int x = threadIdx.x; x < mlpdev::kParams; x += 256) wsh[x] = la.wbf[x];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const size_t pp = S.pitch;
+    V *__restrict__ R = S.rest;
+    mlpdev::InferScratch &sc = isc[wv];
+    // one 16-row tile per wavefront and iteration: lanes 0..15 own the entries
+    for (unsigned int base = (blockIdx.x * 4 + wv) * 16u; base < n; base += gridDim.x * 64u) {
+        const unsigned int i = base + lane;
+        const bool have = lane < 16 && i < n;
+        MapEntry e{};
+        if (have) e = q[i];
+        const CaseParams &cp = cases[have ? e.cs : 0];
+        const KParams k = kp_of(cp, err);
+        double F00 = have ? e.F[0] : 1.0, F01 = e.F[1], F10 = e.F[2], F11 = have ? e.F[3] : 1.0;
+        double Jp = have ? e.Jp : 1.0;
+        const Trial r = compute_pq(F00, F01, F10, F11, cp.mu, cp.kappa);
+        double qlim = 0;
+        const int br = map_branch(k, r, Jp, qlim);
+        const bool need = have && br != kBrInside && r.svd.s1 > 0;
+        double pred[3] = {0, 0, 0};
+        if (__ballot(need)) {
+            for (int x = lane; x < 16 * mlpdev::kIn; x += 64) sc.x[x / mlpdev::kIn][x % mlpdev::kIn] = 0;
+            mlpdev::wave_fence();
+            if (need) {
+                double f[5];
+                map_features(r, Jp, br, f);
+                for (int t = 0; t < 5; ++t)
+                    sc.x[lane][t] = mlpdev::to_bf16((float)((f[t] - la.norm.in_mean[t]) / la.norm.in_scale[t]));
+            }
+            mlpdev::wave_fence();
+            mlpdev::infer16(sc, wsh, la.pf);
+            if (need)
+                for (int t = 0; t < 3; ++t)
+                    pred[t] = (double)sc.y[lane][t] * la.norm.out_scale[t] + la.norm.out_mean[t];
+            mlpdev::wave_fence();
+        }
+        bool accepted = false;
+        if (need) {
+            const double s0 = exp(log(r.svd.s0) + pred[0]);
+            const double s1 = exp(log(r.svd.s1) + pred[1]);
+            // the shear branch leaves Jp_inv unchanged
+            const double Jpn = (br == kBrTension) ? Jp * exp(pred[2]) : Jp;
+            if (yield_residual(k, r, Jp, br, qlim, s0, s1, Jpn) <= la.tol) {
+                usvt(r.svd, s0, s1, F00, F01, F10, F11);
+                Jp = Jpn;
+                accepted = true;
+            }
+        }
+        if (have && !accepted) dp_return(k, r, F00, F01, F10, F11, Jp);
+        const unsigned long long acc = __ballot(accepted), fb = __ballot(have && !accepted);
+        if (lane == 0) {
+            if (acc) atomicAdd(&la.stats[0], (unsigned long long)__popcll(acc));
+            if (fb) atomicAdd(&la.stats[1], (unsigned long long)__popcll(fb));
+        }
+        if (have) {
+            const int pt = e.pt;
+            R[pp * 2 + pt] = (V)(F00 - fe0);
+            R[pp * 3 + pt] = (V)F01;
+            R[pp * 4 + pt] = (V)F10;
+            R[pp * 5 + pt] = (V)(F11 - fe0);
+            if (Jp != e.Jp) R[pp * 10 + pt] = (V)Jp;
+            if (gridOut) scatter_point<F32>(S, cp, k, pt, F00, F01, F10, F11, gridOut, gpitch);
         }
     }
 }
@@ -641,11 +776,20 @@ struct icempm_batch {
     unsigned long long *hcount = nullptr;
     int hcap = 0;
     unsigned long long *stats = nullptr;
+    MapEntry *mapQ = nullptr;          // deferred points (learned map), nPts entries
+    unsigned int *mapCount = nullptr;  // two counters, used alternately
 
     hipGraphExec_t graph = nullptr;
     int graphSteps = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float lastMs = 0;
+
+    // device timing of the fused particle pass (stream launches only)
+    int timingStride = 0;
+    std::vector<hipEvent_t> tev;   // pairs
+    size_t tevNext = 0;
+    double timedMs = 0;
+    int timedLaunches = 0;
 };
 
 namespace {
@@ -667,36 +811,47 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+constexpr int kMapBlocks = 256;
+
 template <bool F32, bool G2P, bool P2G, int RMAP>
-void launch_one(icempm_batch *b, unsigned nb, const double *gV, double *gO, const LearnArgs &la) {
+void launch_one(icempm_batch *b, unsigned nb, const double *gV, double *gO, const LearnArgs &la,
+                const MapQueue &mq) {
     Rows<F32> S{b->pos, reinterpret_cast<typename Rows<F32>::V *>(b->rest), b->crushed, b->pitch};
     hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream, S,
-                       b->cases, b->chunkStart, b->chunkCase, b->nChunks, gV, gO, b->gpitch, b->err, la);
+                       b->cases, b->chunkStart, b->chunkCase, b->nChunks, gV, gO, b->gpitch, b->err, la, mq);
+    if constexpr (G2P && RMAP == kMapLearned) {
+        // the deferred points: map, store, and their P2G into the pass's output grid
+        hipLaunchKernelGGL((k_batch_map<F32>), dim3(kMapBlocks), dim3(256), 0, b->stream, S, b->cases, mq.q,
+                           mq.count, mq.count == b->mapCount ? b->mapCount + 1 : b->mapCount, gO, b->gpitch,
+                           b->err, la);
+    }
 }
 
 template <bool F32, int RMAP>
 void launch_map(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const double *gV, double *gO,
-                const LearnArgs &la) {
+                const LearnArgs &la, const MapQueue &mq) {
     if (g2p && p2g)
-        launch_one<F32, true, true, RMAP>(b, nb, gV, gO, la);
+        launch_one<F32, true, true, RMAP>(b, nb, gV, gO, la, mq);
     else if (g2p)
-        launch_one<F32, true, false, RMAP>(b, nb, gV, gO, la);
+        launch_one<F32, true, false, RMAP>(b, nb, gV, gO, la, mq);
     else
-        launch_one<F32, false, true, kMapExact>(b, nb, gV, gO, la);
+        launch_one<F32, false, true, kMapExact>(b, nb, gV, gO, la, mq);
 }
 
 template <bool F32>
 void launch_prec(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const double *gV, double *gO,
-                 const LearnArgs &la) {
+                 const LearnArgs &la, const MapQueue &mq) {
     if (b->rmap == kMapHarvest)
-        launch_map<F32, kMapHarvest>(b, nb, g2p, p2g, gV, gO, la);
+        launch_map<F32, kMapHarvest>(b, nb, g2p, p2g, gV, gO, la, mq);
     else if (b->rmap == kMapLearned)
-        launch_map<F32, kMapLearned>(b, nb, g2p, p2g, gV, gO, la);
+        launch_map<F32, kMapLearned>(b, nb, g2p, p2g, gV, gO, la, mq);
     else
-        launch_map<F32, kMapExact>(b, nb, g2p, p2g, gV, gO, la);
+        launch_map<F32, kMapExact>(b, nb, g2p, p2g, gV, gO, la, mq);
 }
 
-hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const double *gV, double *gO) {
+// One particle pass (and, with the learned map, the map kernel after it);
+// parity selects the queue counter the pass appends to.
+hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const double *gV, double *gO, int parity) {
     const unsigned nb = (unsigned)(((b->nChunks + 7) / 8) * 8);
     if (b->nChunks <= 0) return hipSuccess;
     LearnArgs la{};
@@ -709,10 +864,11 @@ hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const double *gV, do
     la.hcount = b->hcount;
     la.hcap = b->hcap;
     la.stats = b->stats;
+    MapQueue mq{b->mapQ, b->mapCount ? b->mapCount + (parity & 1) : nullptr, b->nPts};
     if (b->precision == ICEMPM_BATCH_F32)
-        launch_prec<true>(b, nb, g2p, p2g, gV, gO, la);
+        launch_prec<true>(b, nb, g2p, p2g, gV, gO, la, mq);
     else
-        launch_prec<false>(b, nb, g2p, p2g, gV, gO, la);
+        launch_prec<false>(b, nb, g2p, p2g, gV, gO, la, mq);
     return hipGetLastError();
 }
 
@@ -721,7 +877,10 @@ hipError_t enqueue_loop(icempm_batch *b, int n) {
     hipError_t e = launch_fill_f64(b->gridA, b->gpitch * 3, 0.0, b->stream);
     if (e != hipSuccess) return e;
     if ((e = launch_fill_f64(b->hist, 2 * (size_t)n * b->hc.size(), 0.0, b->stream)) != hipSuccess) return e;
-    if ((e = launch_pass(b, false, true, b->gridA, b->gridA)) != hipSuccess) return e;
+    if (b->mapCount &&
+        (e = launch_fill_i32(reinterpret_cast<int *>(b->mapCount), 2, 0, b->stream)) != hipSuccess)
+        return e;
+    if ((e = launch_pass(b, false, true, b->gridA, b->gridA, 0)) != hipSuccess) return e;
     double *A = b->gridA, *B = b->gridB;
     const size_t nc = b->hc.size();
     for (int i = 0; i < n; ++i) {
@@ -731,10 +890,26 @@ hipError_t enqueue_loop(icempm_batch *b, int n) {
                            b->hist + 2 * nc * (size_t)i);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (!last) {
-            if ((e = launch_pass(b, true, true, A, B)) != hipSuccess) return e;
+            const bool timeIt = b->timingStride > 0 && i % b->timingStride == 0;
+            hipEvent_t t0 = nullptr, t1 = nullptr;
+            if (timeIt) {
+                if (b->tevNext + 2 > b->tev.size()) {
+                    for (int q = 0; q < 2; ++q) {
+                        hipEvent_t ev;
+                        if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+                        b->tev.push_back(ev);
+                    }
+                }
+                t0 = b->tev[b->tevNext];
+                t1 = b->tev[b->tevNext + 1];
+                b->tevNext += 2;
+                if ((e = hipEventRecord(t0, b->stream)) != hipSuccess) return e;
+            }
+            if ((e = launch_pass(b, true, true, A, B, i)) != hipSuccess) return e;
+            if (timeIt && (e = hipEventRecord(t1, b->stream)) != hipSuccess) return e;
             std::swap(A, B);
         } else {
-            if ((e = launch_pass(b, true, false, A, nullptr)) != hipSuccess) return e;
+            if ((e = launch_pass(b, true, false, A, nullptr, i)) != hipSuccess) return e;
         }
     }
     return hipSuccess;
@@ -769,6 +944,8 @@ void icempm_batch_destroy(icempm_batch *b) {
     dfree(b->cases); dfree(b->chunkStart); dfree(b->chunkCase); dfree(b->nodeBlock);
     dfree(b->err); dfree(b->tTab); dfree(b->hist);
     dfree(b->mlpW); dfree(b->mlpP); dfree(b->hx); dfree(b->ht); dfree(b->hcount); dfree(b->stats);
+    dfree(b->mapQ); dfree(b->mapCount);
+    for (hipEvent_t ev : b->tev) (void)hipEventDestroy(ev);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -939,7 +1116,9 @@ int icempm_batch_run(icempm_batch *b, int n, int use_graph) {
     }
     BCHK(b, hipMemcpyAsync(b->tTab, tt.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream), "batch run");
     BCHK(b, hipStreamSynchronize(b->stream), "batch run");
-    BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
+    if (use_graph && b->timingStride > 0)
+        return bfail(b, ICEMPM_E_STATE, "run: kernel timing needs stream launches (use_graph = 0)");
+    b->tevNext = 0;
     if (use_graph) {
         if (!b->graph || b->graphSteps != n) {
             if (b->graph) (void)hipGraphExecDestroy(b->graph);
@@ -955,13 +1134,23 @@ int icempm_batch_run(icempm_batch *b, int n, int use_graph) {
             BCHK(b, e, "batch graph instantiate");
             b->graphSteps = n;
         }
+        // timed from here: capture and instantiation are host work before it
+        BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
         BCHK(b, hipGraphLaunch(b->graph, b->stream), "batch graph launch");
     } else {
+        BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
         BCHK(b, enqueue_loop(b, n), "batch run");
     }
     BCHK(b, hipEventRecord(b->e1, b->stream), "batch run");
     BCHK(b, hipEventSynchronize(b->e1), "batch run");
     BCHK(b, hipEventElapsedTime(&b->lastMs, b->e0, b->e1), "batch run");
+    for (size_t q = 0; q + 1 < b->tevNext; q += 2) {
+        float ms = 0;
+        BCHK(b, hipEventElapsedTime(&ms, b->tev[q], b->tev[q + 1]), "batch timing");
+        b->timedMs += ms;
+        b->timedLaunches += 1;
+    }
+    b->tevNext = 0;
     b->t = t;
     b->step += n;
     b->lastRun = n;
@@ -975,6 +1164,23 @@ int icempm_batch_last_ms(const icempm_batch *b, float *ms) {
 }
 
 int icempm_batch_num_points(const icempm_batch *b) { return b ? b->nPts : 0; }
+
+long icempm_batch_num_nodes(const icempm_batch *b) { return b ? b->nNodes : 0; }
+
+int icempm_batch_kernel_timing(icempm_batch *b, int stride) {
+    if (!b || stride < 0) return ICEMPM_E_ARG;
+    b->timingStride = stride;
+    b->timedMs = 0;
+    b->timedLaunches = 0;
+    return ICEMPM_OK;
+}
+
+int icempm_batch_kernel_timing_read(icempm_batch *b, double *total_ms, int *launches) {
+    if (!b || !total_ms || !launches) return ICEMPM_E_ARG;
+    *total_ms = b->timedMs;
+    *launches = b->timedLaunches;
+    return ICEMPM_OK;
+}
 
 int icempm_batch_force_history(icempm_batch *b, int c, double *out, int n) {
     if (!b || !out || c < 0 || (size_t)c >= b->hc.size() || n < 0 || n > b->lastRun) return ICEMPM_E_ARG;
@@ -1080,6 +1286,8 @@ int icempm_batch_set_return_map(icempm_batch *b, int mode, const float *params, 
         for (int q = 0; q < 5; ++q)
             if (!(b->norm.in_scale[q] > 0)) return bfail(b, ICEMPM_E_ARG, "set_return_map: input scale must be > 0");
         b->tol = tol;
+        if (!b->mapQ) BCHK(b, hipMalloc(&b->mapQ, (size_t)b->nPts * sizeof(MapEntry)), "set_return_map");
+        if (!b->mapCount) BCHK(b, hipMalloc(&b->mapCount, 2 * sizeof(unsigned int)), "set_return_map");
     }
     b->rmap = mode;
     return ICEMPM_OK;

@@ -33,9 +33,8 @@ struct KParams {
     int nPts, gridX, gridY, gridTotal, nbins;
     double dt, vol, mass, h, h_inv, Dinv, mu, kappa, gravity, indRsq, indV, vmax;
     double ics, its, iss, grainVar, tanPhi, dpThr;
-    // with GrainVariability == 0 every grain has the constants of grain 0:
-    // they travel here and the table is not read
-    int uniformGrain;
+    // NACC constants of grain 0 (kGPmax.. order): every grain's when
+    // GrainVariability == 0
     double g0[5];
 };
 
@@ -151,6 +150,90 @@ static __device__ __forceinline__ Trial compute_pq(double F00, double F01, doubl
     return r;
 }
 
+// FAST-mode forms of the same quantities (not bit-identical to the
+// reference's operation order): one reciprocal for the polar rotation, and
+// q = 2 |sh0| (in 2D the deviator has sh1 = -sh0, so sqrt2 |sh| = 2 |sh0|).
+static __device__ __forceinline__ Svd2 svd2x2_fast(double F00, double F01, double F10, double F11) {
+    double x0 = F00 + F11;
+    double x1 = F10 - F01;
+    double den2 = x0 * x0 + x1 * x1;
+    double rc = 1.0, rs = 0.0;
+    if (den2 != 0) {
+        const double inv = 1.0 / sqrt(den2);
+        rc = x0 * inv;
+        rs = -x1 * inv;
+    }
+    double S00 = rc * F00 - rs * F10;
+    double S01 = rc * F01 - rs * F11;
+    double S11 = rs * F01 + rc * F11;
+    double cosine, sine, s0, s1;
+    const double y2 = S01 * S01;
+    if (y2 == 0) {
+        cosine = 1;
+        sine = 0;
+        s0 = S00;
+        s1 = S11;
+    } else {
+        double tau = 0.5 * (S00 - S11);
+        double w = sqrt(tau * tau + y2);
+        double t = S01 / ((tau > 0) ? (tau + w) : (tau - w));
+        cosine = 1.0 / sqrt(t * t + 1.0);
+        sine = -t * cosine;
+        double c2 = cosine * cosine;
+        double csy = 2 * cosine * sine * S01;
+        double s2 = sine * sine;
+        s0 = c2 * S00 - csy + s2 * S11;
+        s1 = s2 * S00 + csy + c2 * S11;
+    }
+    Svd2 r;
+    if (s0 < s1) {
+        r.s0 = s1;
+        r.s1 = s0;
+        r.vc = -sine;
+        r.vs = cosine;
+    } else {
+        r.s0 = s0;
+        r.s1 = s1;
+        r.vc = cosine;
+        r.vs = sine;
+    }
+    r.uc = rc * r.vc - rs * r.vs;
+    r.us = rs * r.vc + rc * r.vs;
+    return r;
+}
+
+static __device__ __forceinline__ Trial compute_pq_fast(double F00, double F01, double F10, double F11,
+                                                        double mu, double kappa) {
+    Trial r;
+    r.svd = svd2x2_fast(F00, F01, F10, F11);
+    r.Je = r.svd.s0 * r.svd.s1;
+    r.p = -(kappa / 2.) * (r.Je * r.Je - 1.);
+    r.ss0 = r.svd.s0 * r.svd.s0;
+    r.ss1 = r.svd.s1 * r.svd.s1;
+    const double c = mu / r.Je;
+    r.sh0 = c * (0.5 * (r.ss0 - r.ss1));
+    r.sh1 = -r.sh0;
+    r.q = 2.0 * fabs(r.sh0);
+    return r;
+}
+
+// Kirchhoff stress of the state a Trial describes (its F unchanged since):
+// tau = U diag(sh_i + kappa (J^2 - 1)) U^T = U diag(sh_i - 2 p) U^T.
+static __device__ __forceinline__ void kirchhoff_from_trial(const Trial &r, double &t00, double &t01,
+                                                            double &t11) {
+    const double d0 = r.sh0 - 2. * r.p, d1 = r.sh1 - 2. * r.p;
+    const double cc = r.svd.uc * r.svd.uc, ss = r.svd.us * r.svd.us, cs = r.svd.uc * r.svd.us;
+    t00 = cc * d0 + ss * d1;
+    t01 = cs * (d1 - d0);
+    t11 = ss * d0 + cc * d1;
+}
+
+// NACC test with the constants of grain 0 (GrainVariability == 0)
+static __device__ __forceinline__ bool fails_g0(const KParams &k, double p, double q) {
+    if (p < 0) return (p < -3e6) || (q > k.g0[4] * (p - -3e6));
+    return k.g0[3] * q * q + k.g0[1] * (p + k.g0[2]) * (p - k.g0[0]) > 0;
+}
+
 // Per-grain constants of the NACC test, one row per short grain id
 // (index grain + 32768), computed on the host in the order of
 // GetParametersForGrain (gpu_implementation5.cu:97-111) and
@@ -162,22 +245,15 @@ enum { kGPmax = 0, kGMsq = 1, kGBetaPmax = 2, kGOnePlus2Beta = 3, kGSlope = 4 };
 
 // the same test as fails(), reading the per-grain constants from the table
 static __device__ __forceinline__ bool fails_tab(const KParams &k, short grain, double p, double q) {
-    double pmax, msq, betaPmax, onePlus2Beta, slope;
-    if (k.uniformGrain) {
-        pmax = k.g0[kGPmax]; msq = k.g0[kGMsq]; betaPmax = k.g0[kGBetaPmax];
-        onePlus2Beta = k.g0[kGOnePlus2Beta]; slope = k.g0[kGSlope];
-    } else {
-        const double *g = k.grainTab + ((int)grain + 32768) * kGrainFields;
-        pmax = g[kGPmax]; msq = g[kGMsq]; betaPmax = g[kGBetaPmax];
-        onePlus2Beta = g[kGOnePlus2Beta]; slope = g[kGSlope];
-    }
+    const double *g = k.grainTab + ((int)grain + 32768) * kGrainFields;
     const double pmin2 = -3e6;
     if (p < 0) {
         if (p < pmin2) return true;
-        const double q_limit = slope * (p - pmin2);
+        const double q_limit = g[kGSlope] * (p - pmin2);
         return q > q_limit;
     }
-    const double y = onePlus2Beta * q * q + msq * (p + betaPmax) * (p - pmax);
+    const double pmax = g[kGPmax];
+    const double y = g[kGOnePlus2Beta] * q * q + g[kGMsq] * (p + g[kGBetaPmax]) * (p - pmax);
     return y > 0;
 }
 

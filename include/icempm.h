@@ -213,9 +213,17 @@ int icempm_batch_finalize(icempm_batch *b);
 /* n substeps of every case from the batch's current time; use_graph != 0
  * captures the loop once per n and replays it */
 int icempm_batch_run(icempm_batch *b, int n, int use_graph);
-/* device time of the last icempm_batch_run (HIP events on the batch stream) */
+/* device time of the last icempm_batch_run (HIP events on the batch stream;
+ * a graph run is timed from its launch, after capture and instantiation) */
 int icempm_batch_last_ms(const icempm_batch *b, float *ms);
 int icempm_batch_num_points(const icempm_batch *b);
+long icempm_batch_num_nodes(const icempm_batch *b);
+/* Device timing of the fused particle pass: with stride > 0, every
+ * stride-th fused launch of a stream run (use_graph = 0; a graph run then
+ * fails) is bracketed by events and the durations are summed; stride 0
+ * switches it off.  Enabling resets the sums. */
+int icempm_batch_kernel_timing(icempm_batch *b, int stride);
+int icempm_batch_kernel_timing_read(icempm_batch *b, double *total_ms, int *launches);
 /* total indenter force (fx, fy) of case c for each substep of the last run */
 int icempm_batch_force_history(icempm_batch *b, int c, double *out, int n);
 /* the state of case c in the reference SoA layout (P, Q and grain zero) */
