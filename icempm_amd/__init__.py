@@ -6,13 +6,13 @@ Native parts (built by ``python -m icempm_amd.build``):
   libicempm_hip.so   kernels + C ABI          (include/icempm.h)
   libicempm_host.so  C++ drop-in surface      (include/icempm_host.h)
 """
-from ._lib import (HIP_EXPORTS, HOST_EXPORTS, MODE_FAST, MODE_ORDERED, IcempmError,
+from ._lib import (H5_EXPORTS, HIP_EXPORTS, HOST_EXPORTS, MODE_FAST, MODE_ORDERED, IcempmError,
                    SimParams, hip, host)
 from .model import (CASES, INDENT_1M6, INDENT_60K, INDENT_TINY, IndentationCase, Model,
                     configure_params, lattice_points, nacc_strengths, soa_views)
 
 __all__ = [
-    "HIP_EXPORTS", "HOST_EXPORTS", "MODE_FAST", "MODE_ORDERED", "IcempmError", "SimParams",
+    "H5_EXPORTS", "HIP_EXPORTS", "HOST_EXPORTS", "MODE_FAST", "MODE_ORDERED", "IcempmError", "SimParams",
     "hip", "host", "CASES", "INDENT_1M6", "INDENT_60K", "INDENT_TINY", "IndentationCase",
     "Model", "configure_params", "lattice_points", "nacc_strengths", "soa_views",
 ]

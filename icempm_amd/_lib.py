@@ -14,12 +14,14 @@ from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int16, c_long
 HERE = os.path.dirname(os.path.abspath(__file__))
 HIP_LIB_PATH = os.path.join(HERE, "libicempm_hip.so")
 HOST_LIB_PATH = os.path.join(HERE, "libicempm_host.so")
+H5_LIB_PATH = os.path.join(HERE, "libicempm_h5.so")
 
 ICEMPM_OK = 0
 ICEMPM_E_HIP = 1
 ICEMPM_E_ARG = 2
 ICEMPM_E_STATE = 3
 ICEMPM_E_NODEV = 4
+ICEMPM_E_IO = 5
 MODE_FAST = 0
 MODE_ORDERED = 1
 TUNE_PASS_KIND = 0
@@ -144,8 +146,14 @@ HOST_EXPORTS = [
     "icempm_model_host_force", "icempm_model_set_mode", "icempm_scene_lattice",
 ]
 
+H5_EXPORTS = [
+    "icempm_model_load_raw_points_file", "icempm_model_save_snapshot", "icempm_model_save_pq",
+    "icempm_h5_write_raw_points",
+]
+
 _hip = None
 _host = None
+_h5 = None
 
 
 def hip():
@@ -260,6 +268,27 @@ def host():
     _sig(L, "icempm_scene_lattice", c_long, c_int, c_int, c_int, c_double, c_double, c_uint64,
          POINTER(c_float), POINTER(c_int16))
     _host = L
+    return L
+
+
+def h5_built() -> bool:
+    return os.path.exists(H5_LIB_PATH)
+
+
+def h5():
+    """libicempm_h5.so (SnapshotManager's HDF5 files) with argument types set."""
+    global _h5
+    if _h5 is not None:
+        return _h5
+    host()
+    L = _load(H5_LIB_PATH)
+    P = c_void_p
+    _sig(L, "icempm_model_load_raw_points_file", c_int, P, c_char_p)
+    _sig(L, "icempm_model_save_snapshot", c_int, P, c_char_p)
+    _sig(L, "icempm_model_save_pq", c_int, P, c_char_p)
+    _sig(L, "icempm_h5_write_raw_points", c_int, c_char_p, POINTER(c_float), POINTER(c_int16), c_int,
+         c_float, c_int)
+    _h5 = L
     return L
 
 

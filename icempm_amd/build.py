@@ -3,6 +3,8 @@
   libicempm_hip.so   HIP kernels + C ABI (include/icempm.h), gfx950 only
   libicempm_host.so  the C++ drop-in surface (icy::SimParams, icy::Model,
                      GPU_Implementation5) over the C ABI; plain g++
+  libicempm_h5.so    SnapshotManager's HDF5 files (when the HDF5 C library is
+                     present); plain g++
 
 Run ``python -m icempm_amd.build`` (or ``__graft_entry__.build()``).  Both
 libraries are written next to this file so that they travel with the tree.
@@ -82,6 +84,30 @@ def build_host() -> str:
     return HOST_LIB
 
 
+H5_LIB = os.path.join(HERE, "libicempm_h5.so")
+HDF5_ROOT = os.environ.get("ICEMPM_HDF5_ROOT", "/opt/conda")
+
+
+def hdf5_available() -> bool:
+    return os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")) and \
+        os.path.exists(os.path.join(HDF5_ROOT, "lib", "libhdf5.so"))
+
+
+def build_h5() -> str | None:
+    """libicempm_h5.so (SnapshotManager's HDF5 files) when the HDF5 C library
+    is present (ICEMPM_HDF5_ROOT, default /opt/conda); None otherwise."""
+    if not hdf5_available():
+        return None
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    tmp = H5_LIB + ".tmp"
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, *HOST_FLAGS, f"-isystem{inc}", "-shared", "-o", tmp, os.path.join(HOST, "snapshot_h5.cpp"),
+          f"-L{HERE}", "-licempm_host", "-licempm_hip", f"-L{lib}", "-lhdf5",
+          f"-Wl,-rpath,$ORIGIN:{lib}", "-pthread"])
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "drop_in_driver.cpp")
 DRIVER_BIN = os.path.join(ROOT, "tests", "cpp", "drop_in_driver")
 
@@ -98,6 +124,9 @@ def build_driver() -> str:
 
 def build_all() -> list[str]:
     out = [build_hip(), build_host()]
+    h5 = build_h5()
+    if h5:
+        out.append(h5)
     if os.path.exists(DRIVER_SRC):
         out.append(build_driver())
     return out
@@ -105,7 +134,7 @@ def build_all() -> list[str]:
 
 def clean() -> None:
     shutil.rmtree(OBJ, ignore_errors=True)
-    for p in (HIP_LIB, HOST_LIB, DRIVER_BIN):
+    for p in (HIP_LIB, HOST_LIB, H5_LIB, DRIVER_BIN):
         if os.path.exists(p):
             os.remove(p)
 

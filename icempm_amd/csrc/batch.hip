@@ -192,6 +192,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
     double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
     double B00 = 0, B01 = 0, B10 = 0, B11 = 0;
     bool defer = false;   // learned map: projection and P2G done by the map kernel
+    Trial rt{};           // FP64 storage: the trial state, and whether F is still its F
+    bool fresh = false;
 
     if constexpr (G2P) {
         double Jp = 1;
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             const double n00 = M00 * F00 + M01 * F10, n01 = M00 * F01 + M01 * F11;
             const double n10 = M10 * F00 + M11 * F10, n11 = M10 * F01 + M11 * F11;
             F00 = n00; F01 = n01; F10 = n10; F11 = n11;
-            r = compute_pq(F00, F01, F10, F11, cp.mu, cp.kappa);
+            r = compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa);
             crushed0 = crushed;
             Jp0 = Jp;
             if (crushed == 0) {
@@ -349,6 +351,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
                 if (ins) atomicAdd(&la.stats[2], (unsigned long long)__popcll(ins));
             }
         }
+        if constexpr (!F32) {
+            rt = r;
+            fresh = ok && crushed == 0;
+        }
         if (ok) {
             S.pos[pt] = px;
             S.pos[pp + pt] = py;
@@ -413,7 +419,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         wsync();
         if (ok) {
             double t00, t01, t11;
-            kirchhoff(F00, F01, F10, F11, cp.mu, cp.kappa, t00, t01, t11);
+            if (fresh)
+                kirchhoff_from_trial(rt, t00, t01, t11);
+            else
+                kirchhoff(F00, F01, F10, F11, cp.mu, cp.kappa, t00, t01, t11);
             const double m = cp.mass, h = cp.h;
             const double sc = cp.dt * cp.vol * cp.Dinv;
             const double A00 = m * B00 - sc * t00, A01 = m * B01 - sc * t01;
@@ -591,7 +600,7 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
         const KParams k = kp_of(cp, err);
         double F00 = have ? e.F[0] : 1.0, F01 = e.F[1], F10 = e.F[2], F11 = have ? e.F[3] : 1.0;
         double Jp = have ? e.Jp : 1.0;
-        const Trial r = compute_pq(F00, F01, F10, F11, cp.mu, cp.kappa);
+        const Trial r = compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa);
         double qlim = 0;
         const int br = map_branch(k, r, Jp, qlim);
         const bool need = have && br != kBrInside && r.svd.s1 > 0;

@@ -4,28 +4,9 @@
 #include <stdexcept>
 #include <string>
 
-#include "icempm_host.h"
-#include "model.h"
-#include "snapshotmanager.h"
-
-struct icempm_model {
-    icy::Model model;
-    icy::SnapshotManager snapshot;
-    std::string lastError;
-};
+#include "host_model.h"
 
 namespace {
-
-template <class F>
-int guarded(icempm_model *m, F &&f) {
-    try {
-        f();
-        return ICEMPM_OK;
-    } catch (const std::exception &e) {
-        if (m) m->lastError = e.what();
-        return ICEMPM_E_HIP;
-    }
-}
 
 void copy_str(char *dst, size_t cap, const std::string &s) {
     if (!dst || cap == 0) return;
@@ -66,7 +47,7 @@ int icempm_model_create(icempm_model **out) {
     *out = nullptr;
     icempm_model *m = new icempm_model();
     m->snapshot.model = &m->model;
-    int rc = guarded(m, [&] { m->model.gpu.initialize(); });
+    int rc = icempm_guarded(m, [&] { m->model.gpu.initialize(); });
     if (rc != ICEMPM_OK) {
         delete m;
         return ICEMPM_E_NODEV;
@@ -93,17 +74,17 @@ icempm_ctx *icempm_model_context(icempm_model *m) { return m ? m->model.gpu.cont
 int icempm_model_load_raw_points(icempm_model *m, const float *xy, const int16_t *grains, int n,
                                  float volume) {
     if (!m) return ICEMPM_E_ARG;
-    return guarded(m, [&] { m->snapshot.LoadRawPoints(xy, grains, n, volume); });
+    return icempm_guarded(m, [&] { m->snapshot.LoadRawPoints(xy, grains, n, volume); });
 }
 
 int icempm_model_prepare(icempm_model *m) {
     if (!m) return ICEMPM_E_ARG;
-    return guarded(m, [&] { m->model.Prepare(); });
+    return icempm_guarded(m, [&] { m->model.Prepare(); });
 }
 
 int icempm_model_step(icempm_model *m, int *cont) {
     if (!m) return ICEMPM_E_ARG;
-    return guarded(m, [&] {
+    return icempm_guarded(m, [&] {
         bool r = m->model.Step();
         if (cont) *cont = r ? 1 : 0;
     });
@@ -111,7 +92,7 @@ int icempm_model_step(icempm_model *m, int *cont) {
 
 int icempm_model_wait_cycle(icempm_model *m) {
     if (!m) return ICEMPM_E_ARG;
-    return guarded(m, [&] {
+    return icempm_guarded(m, [&] {
         m->model.gpu.synchronize();
         // the readback callback releases the handshake when it has run
         m->model.processing_current_cycle_data.lock();
@@ -130,7 +111,7 @@ double *icempm_model_host_force(icempm_model *m) {
 
 int icempm_model_set_mode(icempm_model *m, int mode) {
     if (!m) return ICEMPM_E_ARG;
-    return guarded(m, [&] { m->model.gpu.set_mode(mode); });
+    return icempm_guarded(m, [&] { m->model.gpu.set_mode(mode); });
 }
 
 }  // extern "C"

@@ -1,0 +1,221 @@
+// snapshot_h5.cpp -- the HDF5 side of icy::SnapshotManager, on the HDF5 C
+// library: raw point input (snapshotmanager.cpp:186-218), snapshot and P/Q
+// output (snapshotmanager.cpp:14-145), and a writer of raw point files in
+// the reference's input format (so synthetic configurations can be fed to an
+// unchanged frontend).  File layouts (dataset names, types, shapes,
+// attributes) follow the reference's, so its readers and viewers open them.
+#include <hdf5.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <filesystem>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host_model.h"
+#include "icempm_h5.h"
+#include "point.h"
+
+namespace {
+
+// closes an HDF5 handle at scope exit
+struct H {
+    hid_t id;
+    herr_t (*close)(hid_t);
+    H(hid_t i, herr_t (*c)(hid_t)) : id(i), close(c) {}
+    ~H() {
+        if (id >= 0) close(id);
+    }
+    H(const H &) = delete;
+    H &operator=(const H &) = delete;
+    operator hid_t() const { return id; }
+};
+
+void check(herr_t e, const char *what) {
+    if (e < 0) throw std::runtime_error(std::string("HDF5: ") + what);
+}
+hid_t check_id(hid_t id, const char *what) {
+    if (id < 0) throw std::runtime_error(std::string("HDF5: ") + what);
+    return id;
+}
+
+void ensure_dir(const std::string &d) {
+    std::filesystem::path p(d);
+    if (!std::filesystem::is_directory(p)) std::filesystem::create_directory(p);
+}
+
+template <class T>
+void scalar_attr(hid_t obj, const char *name, hid_t type, const T &v) {
+    const hsize_t one = 1;
+    H space(check_id(H5Screate_simple(1, &one, nullptr), "attribute space"), H5Sclose);
+    H att(check_id(H5Acreate2(obj, name, type, space, H5P_DEFAULT, H5P_DEFAULT), name), H5Aclose);
+    check(H5Awrite(att, type, &v), name);
+}
+
+// a contiguous 1D or 2D dataset written in one call
+hid_t write_dataset(hid_t file, const char *name, hid_t type, int rank, const hsize_t *dims,
+                    const void *data) {
+    H space(check_id(H5Screate_simple(rank, dims, nullptr), "dataspace"), H5Sclose);
+    hid_t ds = check_id(H5Dcreate2(file, name, type, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), name);
+    if (H5Dwrite(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, data) < 0) {
+        H5Dclose(ds);
+        throw std::runtime_error(std::string("HDF5: writing ") + name);
+    }
+    return ds;
+}
+
+}  // namespace
+
+void icy::SnapshotManager::SaveParametersAsAttributes(long long dataset)
+{
+    const hid_t ds = (hid_t)dataset;
+    const icy::SimParams &P = model->prms;
+    const int64_t pitch = (int64_t)P.nPtsPitch;
+    scalar_attr(ds, "indenter_x", H5T_NATIVE_DOUBLE, P.indenter_x);
+    scalar_attr(ds, "indenter_y", H5T_NATIVE_DOUBLE, P.indenter_y);
+    scalar_attr(ds, "SimulationTime", H5T_NATIVE_DOUBLE, P.SimulationTime);
+    scalar_attr(ds, "GridX", H5T_NATIVE_INT, P.GridX);
+    scalar_attr(ds, "GridY", H5T_NATIVE_INT, P.GridY);
+    scalar_attr(ds, "nPts", H5T_NATIVE_INT, P.nPts);
+    scalar_attr(ds, "nPtsPitch", H5T_NATIVE_INT64, pitch);
+    scalar_attr(ds, "UpdateEveryNthStep", H5T_NATIVE_INT, P.UpdateEveryNthStep);
+    scalar_attr(ds, "n_indenter_subdivisions", H5T_NATIVE_INT, P.n_indenter_subdivisions);
+    scalar_attr(ds, "cellsize", H5T_NATIVE_DOUBLE, P.cellsize);
+    scalar_attr(ds, "IndDiameter", H5T_NATIVE_DOUBLE, P.IndDiameter);
+    scalar_attr(ds, "InitialTimeStep", H5T_NATIVE_DOUBLE, P.InitialTimeStep);
+    scalar_attr(ds, "SetupType", H5T_NATIVE_INT, P.SetupType);
+    scalar_attr(ds, "Volume", H5T_NATIVE_DOUBLE, P.Volume);
+}
+
+// snapshotmanager.cpp:60-105: <dir>/snapshots/dNNNNN.h5 with the force bins
+// (Indenter_2D), the raw parameter struct (Params, with attributes) and the
+// pitched point SoA (Points), from the pinned readback buffers.
+void icy::SnapshotManager::SaveSnapshot(std::string outputDirectory)
+{
+    if (!model) throw std::runtime_error("SnapshotManager: no model");
+    ensure_dir(outputDirectory);
+    ensure_dir(outputDirectory + "/" + directory_snapshots);
+    char fileName[20];
+    std::snprintf(fileName, sizeof(fileName), "d%05d.h5", model->prms.AnimationFrameNumber());
+    const std::string path = outputDirectory + "/" + directory_snapshots + "/" + fileName;
+
+    H file(check_id(H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), path.c_str()), H5Fclose);
+    const hsize_t nInd = (hsize_t)model->prms.n_indenter_subdivisions * 2;
+    H ind(write_dataset(file, "Indenter_2D", H5T_NATIVE_DOUBLE, 1, &nInd,
+                        model->gpu.host_side_indenter_force_accumulator), H5Dclose);
+    const hsize_t nPrm = sizeof(icy::SimParams);
+    H prm(write_dataset(file, "Params", H5T_NATIVE_B8, 1, &nPrm, &model->prms), H5Dclose);
+    SaveParametersAsAttributes(prm.id);
+    const hsize_t nPts = (hsize_t)model->prms.nPtsPitch * icy::SimParams::nPtsArrays;
+    H pts(write_dataset(file, "Points", H5T_NATIVE_DOUBLE, 1, &nPts, model->gpu.tmp_transfer_buffer), H5Dclose);
+}
+
+// snapshotmanager.cpp:14-56: <dir>/pq/pqNNNNN.h5 with (p, q) of the intact
+// points (PQ_intact, with the parameter attributes) and of the crushed ones
+// (PQ_crushed), in point order.
+void icy::SnapshotManager::SavePQ(std::string outputDirectory)
+{
+    if (!model) throw std::runtime_error("SnapshotManager: no model");
+    ensure_dir(outputDirectory);
+    ensure_dir(outputDirectory + "/" + directory_pq);
+    char fileName[20];
+    std::snprintf(fileName, sizeof(fileName), "pq%05d.h5", model->prms.AnimationFrameNumber());
+    const std::string path = outputDirectory + "/" + directory_pq + "/" + fileName;
+
+    const int n = model->prms.nPts;
+    const size_t pitch = model->prms.nPtsPitch;
+    const double *buf = model->gpu.tmp_transfer_buffer;
+    buffer1.clear();
+    buffer2.clear();
+    buffer1.reserve(2 * (size_t)n);
+    buffer2.reserve(2 * (size_t)n);
+    const unsigned char *u8 = reinterpret_cast<const unsigned char *>(buf);
+    for (int i = 0; i < n; i++) {
+        const double p = buf[pitch * icy::SimParams::idx_P + i];
+        const double q = buf[pitch * icy::SimParams::idx_Q + i];
+        std::vector<double> &dst = u8[i] ? buffer1 : buffer2;
+        dst.push_back(p);
+        dst.push_back(q);
+    }
+    H file(check_id(H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), path.c_str()), H5Fclose);
+    const hsize_t di[2] = {buffer2.size() / 2, 2};
+    H intact(write_dataset(file, "PQ_intact", H5T_NATIVE_DOUBLE, 2, di, buffer2.data()), H5Dclose);
+    SaveParametersAsAttributes(intact.id);
+    const hsize_t dc[2] = {buffer1.size() / 2, 2};
+    H crushed(write_dataset(file, "PQ_crushed", H5T_NATIVE_DOUBLE, 2, dc, buffer1.data()), H5Dclose);
+}
+
+// snapshotmanager.cpp:186-218: the raw point cloud from HDF5, then the
+// array path (sort, placement, upload).
+void icy::SnapshotManager::LoadRawPoints(std::string fileName)
+{
+    if (!model) throw std::runtime_error("SnapshotManager: no model");
+    if (!std::filesystem::exists(fileName)) throw std::runtime_error("error reading raw points file - no file");
+    H file(check_id(H5Fopen(fileName.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT), fileName.c_str()), H5Fclose);
+    H ds(check_id(H5Dopen2(file, "Points_Raw_2D", H5P_DEFAULT), "Points_Raw_2D"), H5Dclose);
+    H space(check_id(H5Dget_space(ds), "Points_Raw_2D space"), H5Sclose);
+    hsize_t dims[2] = {0, 0};
+    if (H5Sget_simple_extent_ndims(space) != 2) throw std::runtime_error("error reading raw points file - dimensions mismatch");
+    H5Sget_simple_extent_dims(space, dims, nullptr);
+    if (dims[1] != (hsize_t)icy::SimParams::dim) throw std::runtime_error("error reading raw points file - dimensions mismatch");
+    const int n = (int)dims[0];
+    std::vector<float> xy(2 * (size_t)n);
+    check(H5Dread(ds, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, xy.data()), "reading Points_Raw_2D");
+    H dg(check_id(H5Dopen2(file, "GrainIDs", H5P_DEFAULT), "GrainIDs"), H5Dclose);
+    std::vector<short> grains((size_t)n);
+    check(H5Dread(dg, H5T_NATIVE_INT16, H5S_ALL, H5S_ALL, H5P_DEFAULT, grains.data()), "reading GrainIDs");
+    H att(check_id(H5Aopen(dg, "volume", H5P_DEFAULT), "volume attribute"), H5Aclose);
+    float volume = 0;
+    check(H5Aread(att, H5T_NATIVE_FLOAT, &volume), "reading volume");
+    LoadRawPoints(xy.data(), grains.data(), n, volume);
+}
+
+extern "C" {
+
+int icempm_model_load_raw_points_file(icempm_model *m, const char *path) {
+    if (!m || !path) return ICEMPM_E_ARG;
+    return icempm_guarded(m, [&] { m->snapshot.LoadRawPoints(std::string(path)); });
+}
+
+int icempm_model_save_snapshot(icempm_model *m, const char *dir) {
+    if (!m || !dir) return ICEMPM_E_ARG;
+    return icempm_guarded(m, [&] { m->snapshot.SaveSnapshot(std::string(dir)); });
+}
+
+int icempm_model_save_pq(icempm_model *m, const char *dir) {
+    if (!m || !dir) return ICEMPM_E_ARG;
+    return icempm_guarded(m, [&] { m->snapshot.SavePQ(std::string(dir)); });
+}
+
+int icempm_h5_write_raw_points(const char *path, const float *xy, const int16_t *grains, int n, float volume,
+                               int deflate) {
+    if (!path || !xy || !grains || n <= 0 || deflate < 0 || deflate > 9) return ICEMPM_E_ARG;
+    try {
+        H file(check_id(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), path), H5Fclose);
+        const hsize_t d2[2] = {(hsize_t)n, 2}, d1 = (hsize_t)n;
+        H plist2(check_id(H5Pcreate(H5P_DATASET_CREATE), "plist"), H5Pclose);
+        H plist1(check_id(H5Pcreate(H5P_DATASET_CREATE), "plist"), H5Pclose);
+        if (deflate > 0) {
+            const hsize_t c2[2] = {(hsize_t)std::min(n, 1 << 16), 2}, c1 = (hsize_t)std::min(n, 1 << 16);
+            check(H5Pset_chunk(plist2, 2, c2), "chunk");
+            check(H5Pset_deflate(plist2, (unsigned)deflate), "deflate");
+            check(H5Pset_chunk(plist1, 1, &c1), "chunk");
+            check(H5Pset_deflate(plist1, (unsigned)deflate), "deflate");
+        }
+        H s2(check_id(H5Screate_simple(2, d2, nullptr), "space"), H5Sclose);
+        H ds(check_id(H5Dcreate2(file, "Points_Raw_2D", H5T_IEEE_F32LE, s2, H5P_DEFAULT, plist2, H5P_DEFAULT),
+                      "Points_Raw_2D"), H5Dclose);
+        check(H5Dwrite(ds, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, xy), "writing Points_Raw_2D");
+        H s1(check_id(H5Screate_simple(1, &d1, nullptr), "space"), H5Sclose);
+        H dg(check_id(H5Dcreate2(file, "GrainIDs", H5T_STD_I16LE, s1, H5P_DEFAULT, plist1, H5P_DEFAULT),
+                      "GrainIDs"), H5Dclose);
+        check(H5Dwrite(dg, H5T_NATIVE_INT16, H5S_ALL, H5S_ALL, H5P_DEFAULT, grains), "writing GrainIDs");
+        scalar_attr(dg, "volume", H5T_IEEE_F32LE, volume);
+        return ICEMPM_OK;
+    } catch (const std::exception &) {
+        return ICEMPM_E_IO;
+    }
+}
+
+}  // extern "C"

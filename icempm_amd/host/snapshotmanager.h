@@ -1,8 +1,10 @@
-// snapshotmanager.h -- the ingest half of icy::SnapshotManager
-// (reference: simulation/snapshotmanager.h:13-28, snapshotmanager.cpp:186-304).
-// LoadRawPoints takes the raw point cloud as arrays (the reference reads the
-// same data -- Points_Raw_2D f32[N][2], GrainIDs i16[N], a volume attribute --
-// from HDF5); SortPoints runs the stable cell-key sort on the device.
+// snapshotmanager.h -- icy::SnapshotManager (reference:
+// simulation/snapshotmanager.h:13-28, snapshotmanager.cpp:14-304).
+// LoadRawPoints on arrays and SortPoints (the stable cell-key sort on the
+// device) are in libicempm_host.so; the HDF5 members -- LoadRawPoints from a
+// file (Points_Raw_2D f32[N][2], GrainIDs i16[N] with a f32 volume
+// attribute), SaveSnapshot, SavePQ -- are in libicempm_h5.so
+// (snapshot_h5.cpp), which links the HDF5 C library.
 #ifndef SNAPSHOTMANAGER_H
 #define SNAPSHOTMANAGER_H
 
@@ -19,6 +21,18 @@ public:
 
     void LoadRawPoints(const float *xy, const short *grains, int nPoints, float volume);
     void SortPoints(std::vector<std::tuple<float, float, short>> &buffer);
+
+    // libicempm_h5.so
+    void LoadRawPoints(std::string fileName);
+    void SaveSnapshot(std::string directory);
+    void SavePQ(std::string directory);
+    // the attributes SaveSnapshot and SavePQ attach to a dataset (an HDF5
+    // hid_t; the reference takes an H5::DataSet)
+    void SaveParametersAsAttributes(long long dataset);
+
+    const std::string directory_snapshots = "snapshots";
+    const std::string directory_pq = "pq";
+    std::vector<double> buffer1, buffer2;
 };
 
 #endif

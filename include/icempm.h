@@ -40,7 +40,8 @@ enum {
     ICEMPM_E_HIP = 1,      /* a HIP runtime call failed */
     ICEMPM_E_ARG = 2,      /* bad argument (null, size mismatch, out of range) */
     ICEMPM_E_STATE = 3,    /* call out of order (e.g. step before allocate) */
-    ICEMPM_E_NODEV = 4     /* no HIP device */
+    ICEMPM_E_NODEV = 4,    /* no HIP device */
+    ICEMPM_E_IO = 5        /* file input/output (libicempm_h5.so) */
 };
 
 /* P2G summation modes (icempm_set_mode).  FAST accumulates in LDS and then

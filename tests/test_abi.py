@@ -8,7 +8,7 @@ import re
 import pytest
 
 import oracle
-from icempm_amd import HIP_EXPORTS, HOST_EXPORTS, SimParams, _lib, hip, host
+from icempm_amd import H5_EXPORTS, HIP_EXPORTS, HOST_EXPORTS, SimParams, _lib, hip, host
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -21,9 +21,16 @@ def declared(header: str) -> list[str]:
     return sorted(set(n for n in names if not n.startswith("icempm_callback")))
 
 
+def h5_loader():
+    if not _lib.h5_built():
+        pytest.skip("libicempm_h5.so not built (no HDF5 C library)")
+    return _lib.h5()
+
+
 @pytest.mark.parametrize("header,loader,listed", [
     ("icempm.h", hip, HIP_EXPORTS),
     ("icempm_host.h", host, HOST_EXPORTS),
+    ("icempm_h5.h", h5_loader, H5_EXPORTS),
 ])
 def test_exports(header, loader, listed):
     names = declared(header)

@@ -70,3 +70,37 @@ def test_training_reduces_loss(gpu_available):
     last, _ = m.step(dX, dT, 0, 65536, 0.0)
     assert last < first / 10, (first, last)
     m.close()
+
+
+def bf16(a):
+    """float32 -> bfloat16 -> float32, round to nearest even (finite inputs)."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def test_inference_matches_bf16_emulation(gpu_available):
+    """BF16 operands (inputs, weights, hidden activations), FP32 sums and
+    biases: a numpy emulation agrees to FP32 summation-order differences."""
+    print("mlp infer bf16", flush=True)
+    rng = np.random.default_rng(8)
+    P = init_params(9)
+    P[1024:1088] = rng.normal(0, 0.1, 64)
+    P[5184:5248] = rng.normal(0, 0.1, 64)
+    P[6272:6275] = rng.normal(0, 0.1, 3)
+    X, _ = data(rng, 333)
+    m = MLP(P)
+    Y = m.infer(DeviceArray(X), 333)
+    m.close()
+    f = np.float32
+    W1 = bf16(P[0:1024].reshape(64, 16)[:, :5])
+    W2 = bf16(P[1088:5184].reshape(64, 64))
+    W3 = bf16(P[5248:6272].reshape(16, 64)[:3])
+    h1 = bf16(np.tanh((bf16(X) @ W1.T).astype(f) + P[1024:1088].astype(f)))
+    h2 = bf16(np.tanh((h1 @ W2.T).astype(f) + P[5184:5248].astype(f)))
+    ref = (h2 @ W3.T).astype(f) + P[6272:6275].astype(f)
+    # a hidden activation within an FP32 rounding of a BF16 tie can round the
+    # other way; allow a few such flips
+    err = np.abs(Y - ref)
+    assert np.quantile(err, 0.99) <= 1e-4 * max(1.0, np.abs(ref).max())
+    assert err.max() <= 2e-2 * max(1.0, np.abs(ref).max())

@@ -42,3 +42,24 @@ def test_padding_gets_no_gradient():
     assert not W1[:, 5:].any()                      # inputs 5..15 are zero
     W3 = g[5248:6272].reshape(16, 64)
     assert not W3[3:].any() and not g[6275:6288].any()   # outputs 3..15 carry no loss
+
+
+def test_forward_matches_numpy_layout():
+    """The documented parameter layout (W[out][in], W1 64x16 at 0, b1 at 1024,
+    W2 64x64 at 1088, b2 at 5184, W3 16x64 at 5248, b3 at 6272) evaluated
+    with numpy matrix products."""
+    rng = np.random.default_rng(6)
+    P = init_params(rng)
+    X = rng.normal(size=(9, 5))
+    T = np.zeros((9, 3))
+    _, _, Y = oracle.mlp_loss_grad(P, X, T, False)
+    W1 = P[0:1024].reshape(64, 16)[:, :5]
+    b1 = P[1024:1088]
+    W2 = P[1088:5184].reshape(64, 64)
+    b2 = P[5184:5248]
+    W3 = P[5248:6272].reshape(16, 64)[:3]
+    b3 = P[6272:6275]
+    h1 = np.tanh(X @ W1.T + b1)
+    h2 = np.tanh(h1 @ W2.T + b2)
+    ref = h2 @ W3.T + b3
+    assert np.allclose(Y, ref, rtol=1e-12, atol=1e-12)
