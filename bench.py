@@ -88,7 +88,7 @@ def cpu_baseline(case, pts0, prms0, target_s: float, threads: int):
     t0 = time.perf_counter()
     oracle.run(p, pts, grid, bins, 1, nthreads=threads)
     one = time.perf_counter() - t0
-    nsub = max(1, min(2000, int(target_s / max(one, 1e-6))))
+    nsub = max(1, min(20000, int(target_s / max(one, 1e-6))))
     t0 = time.perf_counter()
     _, err = oracle.run(p, pts, grid, bins, nsub, nthreads=threads)
     el = time.perf_counter() - t0

@@ -232,13 +232,36 @@ int build_chunks(icempm_ctx *c) {
         starts.push_back(0);
         auto rowof = [gx](int key) { return key >= 0 ? key / gx : -1; };
         auto colof = [gx](int key) { return key >= 0 ? key % gx : -1; };
-        for (int i = 1; i < n; ++i) {
-            const int kc = hk[(size_t)cs], ki = hk[(size_t)i];
-            const bool brk = (i - cs >= kChunk) || rowof(ki) != rowof(kc) ||
-                             colof(ki) < colof(kc) || colof(ki) > colof(kc) + (kWinX - 3);
-            if (brk) {
-                starts.push_back(i);
-                cs = i;
+        if (c->passKind == 0) {
+            // workgroup windows: one cell row, columns ascending within the window
+            for (int i = 1; i < n; ++i) {
+                const int kc = hk[(size_t)cs], ki = hk[(size_t)i];
+                const bool brk = (i - cs >= kChunk) || rowof(ki) != rowof(kc) ||
+                                 colof(ki) < colof(kc) || colof(ki) > colof(kc) + (kWinX - 3);
+                if (brk) {
+                    starts.push_back(i);
+                    cs = i;
+                }
+            }
+        } else {
+            // wavefront windows (4 rows): at most two cell rows and 126 columns per
+            // chunk.  The ingest sorts by the cell of the raw coordinates; after
+            // the block offset the points of one raw row can interleave two rows.
+            int r0 = rowof(hk[0]), r1 = r0, c0 = colof(hk[0]), c1 = c0;
+            for (int i = 1; i < n; ++i) {
+                const int ki = hk[(size_t)i], r = rowof(ki), cc = colof(ki);
+                const int nr0 = std::min(r0, r), nr1 = std::max(r1, r);
+                const int nc0 = std::min(c0, cc), nc1 = std::max(c1, cc);
+                const bool brk = (i - cs >= kChunk) || nr1 - nr0 > 1 || nc1 - nc0 > 125 || ki < 0 ||
+                                 hk[(size_t)cs] < 0;
+                if (brk) {
+                    starts.push_back(i);
+                    cs = i;
+                    r0 = r1 = r;
+                    c0 = c1 = cc;
+                } else {
+                    r0 = nr0; r1 = nr1; c0 = nc0; c1 = nc1;
+                }
             }
         }
         // unsorted input: fixed chunks (correct, P2G then falls back to
@@ -772,6 +795,7 @@ int icempm_set_tuning(icempm_ctx *c, int knob, int value) {
     switch (knob) {
     case ICEMPM_TUNE_PASS_KIND:
         if (value < 0 || value > 2) return fail(c, ICEMPM_E_ARG, "set_tuning: pass kind is 0, 1 or 2");
+        if ((value == 0) != (c->passKind == 0)) c->chunksDirty = true;   // the chunk rule differs
         c->passKind = value;
         return ICEMPM_OK;
     default:

@@ -40,7 +40,7 @@ struct CaseParams {
     int gx, gy, gridTotal, nPts, ptOffset, gridOffset, pad0, pad1;
 };
 
-constexpr int kWX = 48, kWS = 48, kWY = 4;
+constexpr int kWX = 48, kWS = 52, kWY = 4;   // window width, padded row stride, rows
 
 template <bool F32>
 struct Rows {
@@ -411,7 +411,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         const int oy = wmin(ok ? by : INT_MAX);
         double *W0 = smem[wv];
         double *W1 = W0 + kWY * kWS, *W2 = W1 + kWY * kWS;
-        for (int e = lane; e < kWS * kWY; e += 64) {
+        for (int c2 = lane; c2 < kWX * kWY; c2 += 64) {
+            const int e = (c2 / kWX) * kWS + c2 % kWX;
             W0[e] = 0;
             W1[e] = 0;
             W2[e] = 0;
@@ -493,11 +494,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         }
         wsync();
         if (ox != INT_MAX) {
-            for (int e = lane; e < kWS * kWY; e += 64) {
+            for (int c2 = lane; c2 < kWX * kWY; c2 += 64) {
+                const int lx = c2 % kWX, ly = c2 / kWX;
+                const int e = ly * kWS + lx;
                 const double a = W0[e];
-                const int lx = e % kWS;
-                if (a != 0 && lx < kWX) {
-                    const int X = ox + lx, Y = oy + e / kWS;
+                if (a != 0) {
+                    const int X = ox + lx, Y = oy + ly;
                     if (X < gx && Y < gy) {
                         const int node = X + Y * gx;
                         unsafeAtomicAdd(&gO[node], a);
@@ -1024,17 +1026,28 @@ int icempm_batch_finalize(icempm_batch *b) {
         h.cp.ptOffset = (int)pofs;
         h.cp.gridOffset = (int)gofs;
         const int n = h.cp.nPts, gx = h.cp.gx;
+        // chunks of <= 256 consecutive points whose base cells span at most two
+        // cell rows and 126 columns: a wavefront's window (4 rows) then holds
+        // all its stencils.  Two rows, not one: the ingest sorts by the cell of
+        // the raw coordinates, and after the block offset the points of one raw
+        // row can interleave two rows (the first raw row always does).
         int cs = 0;
         cstart.push_back((int)pofs);
         ccase.push_back((int)ci);
+        int r0 = h.keys[0] / gx, r1 = r0, c0 = h.keys[0] % gx, c1 = c0;
         for (int i = 1; i < n; ++i) {
-            const int kc = h.keys[cs], ki = h.keys[i];
-            const bool brk = (i - cs >= 256) || (ki / gx != kc / gx) || (ki % gx < kc % gx) ||
-                             (ki % gx > kc % gx + 40) || kc < 0 || ki < 0;
+            const int ki = h.keys[i];
+            const int r = ki / gx, cc = ki % gx;
+            const int nr0 = std::min(r0, r), nr1 = std::max(r1, r), nc0 = std::min(c0, cc), nc1 = std::max(c1, cc);
+            const bool brk = (i - cs >= 256) || nr1 - nr0 > 1 || nc1 - nc0 > 125 || ki < 0 || h.keys[cs] < 0;
             if (brk) {
                 cstart.push_back((int)(pofs + i));
                 ccase.push_back((int)ci);
                 cs = i;
+                r0 = r1 = r;
+                c0 = c1 = cc;
+            } else {
+                r0 = nr0; r1 = nr1; c0 = nc0; c1 = nc1;
             }
         }
         for (int nd = 0; nd < h.cp.gridTotal; nd += 256) nblk.push_back(make_int2((int)ci, nd));
