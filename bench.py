@@ -259,21 +259,21 @@ def run_batch_learned(args) -> dict:
     train_s = time.perf_counter() - t0
     log(f"bench: trained {args.train_steps} Adam steps in {train_s:.1f} s, loss {losses[-1][1]:.3e}")
 
-    # 3. plain batched engines
-    b = build_batch(cases, BATCH_F64, states)
-    f64_pups, _, _ = timed(b, "batched FP64 exact")
-    b.close()
-    b = build_batch(cases, BATCH_F32, states)
-    f32_pups, _, f32_hist = timed(b, "batched FP32 exact")
-    b.close()
-
-    # 4. the measured configuration: batched FP32 + learned map
+    # 3. the measured configuration: batched FP32 + learned map, autotuned
+    #    (points per chunk and re-sort cadence, timed from a checkpoint)
     b = build_batch(cases, BATCH_F32, states)
     b.set_learned(params, norm, tol=args.tol)
+    tune_rows = []
+    if args.autotune_steps > 0:
+        t0 = time.perf_counter()
+        tune_rows = b.autotune(args.autotune_steps)
+        b.map_stats(reset=True)
+        log(f"bench: autotune {time.perf_counter() - t0:.1f} s -> chunk {b.tuning()[0]}, "
+            f"re-sort every {b.tuning()[1]}: " + ", ".join(f"({c},{r}) {t * 1e3:.0f} us" for c, r, t in tune_rows))
+    chunk, resort_every = b.tuning()
     value, ms, hist = timed(b, "batched FP32 learned")
     acc, fb, ins = b.map_stats(reset=True)
     errs = [rel_l2(h, r) for h, r in zip(hist, ref_hist)]
-    errs32 = [rel_l2(h, r) for h, r in zip(f32_hist, ref_hist)]
     # the fused pass alone, timed with events on the batch stream
     b.kernel_timing(args.timing_stride)
     b.run(args.kernel_steps, use_graph=False)
@@ -285,6 +285,17 @@ def run_batch_learned(args) -> dict:
     achieved = k_bytes / k_avg_s / 1e9 if klaunch else None
     log(f"bench: fused pass {k_avg_s * 1e6:.1f} us ({klaunch} timed launches), "
         f"{achieved:.0f} GB/s algorithmic")
+
+    # 4. the batched engine with the exact map, same setting
+    b = build_batch(cases, BATCH_F64, states)
+    b.set_tuning(chunk=chunk, resort_every=resort_every)
+    f64_pups, _, _ = timed(b, "batched FP64 exact")
+    b.close()
+    b = build_batch(cases, BATCH_F32, states)
+    b.set_tuning(chunk=chunk, resort_every=resort_every)
+    f32_pups, _, f32_hist = timed(b, "batched FP32 exact")
+    b.close()
+    errs32 = [rel_l2(h, r) for h, r in zip(f32_hist, ref_hist)]
 
     # 5. the plain MPM: the single-case FP64 engine, case by case
     seq_updates, seq_time = 0, 0.0
@@ -339,6 +350,9 @@ def run_batch_learned(args) -> dict:
             "grid_nodes": nodes,
             "storage": "f32 (positions f64)",
             "mlp": "bf16 weights/activations, f32 accumulation",
+            "points_per_chunk": chunk,
+            "resort_every": resort_every,
+            "autotune": [{"chunk": c, "resort_every": r, "ms_per_substep": t} for c, r, t in tune_rows],
         },
         "roofline": {
             "bound": "hbm",
@@ -397,6 +411,8 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--plain-steps", type=int, default=400,
                     help="batch: substeps per case of the plain single-case comparison")
+    ap.add_argument("--autotune-steps", type=int, default=200,
+                    help="batch: substeps per autotuner candidate (0: keep the defaults)")
     ap.add_argument("--kernel-steps", type=int, default=200,
                     help="batch: stream-launched substeps for the fused-pass timing")
     ap.add_argument("--timing-stride", type=int, default=4,

@@ -30,6 +30,8 @@ BATCH_F32 = 1
 MAP_EXACT = 0
 MAP_HARVEST = 1
 MAP_LEARNED = 2
+BATCH_TUNE_CHUNK = 0
+BATCH_TUNE_RESORT = 1
 
 
 class IcempmError(RuntimeError):
@@ -131,6 +133,8 @@ HIP_EXPORTS = [
     "icempm_batch_force_history", "icempm_batch_read_points", "icempm_batch_error_flag",
     "icempm_batch_set_return_map", "icempm_batch_read_harvest", "icempm_batch_map_stats",
     "icempm_batch_num_nodes", "icempm_batch_kernel_timing", "icempm_batch_kernel_timing_read",
+    "icempm_batch_set_tuning", "icempm_batch_get_tuning", "icempm_batch_resort",
+    "icempm_batch_checkpoint", "icempm_batch_restore", "icempm_batch_autotune",
     "icempm_mlp_num_params", "icempm_mlp_create", "icempm_mlp_destroy", "icempm_mlp_last_error",
     "icempm_mlp_get_params", "icempm_mlp_device_params", "icempm_mlp_step", "icempm_mlp_infer",
     "icempm_mfma_probe", "icempm_device_alloc", "icempm_device_free", "icempm_copy_to_device",
@@ -220,6 +224,12 @@ def hip():
     _sig(L, "icempm_batch_num_nodes", c_long, P)
     _sig(L, "icempm_batch_kernel_timing", c_int, P, c_int)
     _sig(L, "icempm_batch_kernel_timing_read", c_int, P, POINTER(c_double), POINTER(c_int))
+    _sig(L, "icempm_batch_set_tuning", c_int, P, c_int, c_int)
+    _sig(L, "icempm_batch_get_tuning", c_int, P, c_int)
+    _sig(L, "icempm_batch_resort", c_int, P)
+    _sig(L, "icempm_batch_checkpoint", c_int, P)
+    _sig(L, "icempm_batch_restore", c_int, P)
+    _sig(L, "icempm_batch_autotune", c_int, P, c_int, POINTER(c_double), c_int, POINTER(c_int))
     fp = POINTER(c_float)
     _sig(L, "icempm_mlp_num_params", c_int)
     _sig(L, "icempm_mlp_create", c_int, POINTER(c_void_p), fp)

@@ -12,8 +12,8 @@ from ctypes import POINTER, byref, c_double, c_float, c_int, c_uint64, c_void_p
 import numpy as np
 
 from . import _lib
-from ._lib import (BATCH_F32, BATCH_F64, MAP_EXACT, MAP_HARVEST, MAP_LEARNED, IcempmError,
-                   SimParams)
+from ._lib import (BATCH_F32, BATCH_F64, BATCH_TUNE_CHUNK, BATCH_TUNE_RESORT, MAP_EXACT, MAP_HARVEST,
+                   MAP_LEARNED, IcempmError, SimParams)
 from .model import MODE_FAST, IndentationCase, Model, nacc_strengths
 
 
@@ -130,6 +130,36 @@ class Batch:
         f = c_int(0)
         self._check(self._H.icempm_batch_error_flag(self._b, byref(f)))
         return f.value
+
+    # -- tuning, re-sort, checkpoint -------------------------------------------
+    def set_tuning(self, chunk: int | None = None, resort_every: int | None = None) -> None:
+        """Points per chunk (64, 128, 256) and the re-sort cadence in substeps (0: never)."""
+        if chunk is not None:
+            self._check(self._H.icempm_batch_set_tuning(self._b, BATCH_TUNE_CHUNK, int(chunk)))
+        if resort_every is not None:
+            self._check(self._H.icempm_batch_set_tuning(self._b, BATCH_TUNE_RESORT, int(resort_every)))
+
+    def tuning(self) -> tuple[int, int]:
+        return (int(self._H.icempm_batch_get_tuning(self._b, BATCH_TUNE_CHUNK)),
+                int(self._H.icempm_batch_get_tuning(self._b, BATCH_TUNE_RESORT)))
+
+    def resort(self) -> None:
+        self._check(self._H.icempm_batch_resort(self._b))
+
+    def checkpoint(self) -> None:
+        self._check(self._H.icempm_batch_checkpoint(self._b))
+
+    def restore(self) -> None:
+        self._check(self._H.icempm_batch_restore(self._b))
+
+    def autotune(self, probe_steps: int = 200) -> list[tuple[int, int, float]]:
+        """Times every (points per chunk, re-sort cadence) candidate over
+        probe_steps substeps from the current state, keeps the fastest and
+        returns the state to where it was.  Rows: (chunk, re-sort, ms/substep)."""
+        rep = (c_double * 64)()
+        rows = c_int(0)
+        self._check(self._H.icempm_batch_autotune(self._b, int(probe_steps), rep, 21, byref(rows)))
+        return [(int(rep[3 * i]), int(rep[3 * i + 1]), float(rep[3 * i + 2])) for i in range(rows.value)]
 
     # -- return map of crushed points ------------------------------------------
     def set_exact(self) -> None:

@@ -17,6 +17,7 @@
 // variant only stores less: 93 B per point and substep instead of 164 B.
 // Material is uniform per case (one set of NACC constants per case).
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -733,6 +734,45 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
     }
 }
 
+// Cell key of every point of the batch, offset by its case's first node so
+// that a sort keeps the cases contiguous and in order; vals = slot index.
+__global__ void k_batch_keys(const double *__restrict__ pos, size_t pitch, int nPts,
+                             const CaseParams *__restrict__ cases, int nCases, int *__restrict__ keys,
+                             int *__restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPts) return;
+    int lo = 0, hi = nCases - 1;   // the last case with ptOffset <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cases[mid].ptOffset <= i) lo = mid; else hi = mid - 1;
+    }
+    const CaseParams &cp = cases[lo];
+    const double x = pos[i], y = pos[pitch + i];
+    int key = cp.gridTotal - 1;
+    if (x == x && y == y) {
+        const double bx = fmin(fmax(x * cp.h_inv - 0.5, 0.0), (double)(cp.gx - 1));
+        const double by = fmin(fmax(y * cp.h_inv - 0.5, 0.0), (double)(cp.gy - 1));
+        key = (int)bx + (int)by * cp.gx;
+    }
+    keys[i] = cp.gridOffset + key;
+    vals[i] = i;
+}
+
+// Point state permuted: slot k of the destination <- slot perm[k] of the source.
+template <bool F32>
+__global__ void k_batch_permute(Rows<F32> src, Rows<F32> dst, const int *__restrict__ srcOrig,
+                                int *__restrict__ dstOrig, const int *__restrict__ perm, int nPts) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nPts) return;
+    const int j = perm[k];
+    const size_t p = src.pitch;
+    dst.pos[k] = src.pos[j];
+    dst.pos[p + k] = src.pos[p + j];
+    for (int r = 0; r < 11; ++r) dst.rest[r * p + k] = src.rest[r * p + j];
+    dst.crushed[k] = src.crushed[j];
+    dstOrig[k] = srcOrig[j];
+}
+
 }  // namespace batch
 }  // namespace icempm
 
@@ -766,6 +806,7 @@ struct icempm_batch {
     CaseParams *cases = nullptr;
     int *chunkStart = nullptr, *chunkCase = nullptr;
     int nChunks = 0;
+    size_t chunkCapDev = 0;
     int2 *nodeBlock = nullptr;
     int nNodeBlocks = 0;
     int *err = nullptr;
@@ -792,8 +833,34 @@ struct icempm_batch {
 
     hipGraphExec_t graph = nullptr;
     int graphSteps = 0;
+    const double *graphT = nullptr;   // the time table and history rows the graph was captured with
+    double *graphH = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float lastMs = 0;
+
+    // tuning: points per chunk (= threads per workgroup of the particle pass)
+    // and the re-sort cadence in substeps (0: never)
+    int chunkPoints = 256;
+    int resortEvery = 0;
+    std::vector<int> hcstart, hccase;       // the chunk table on the host
+    int *orig = nullptr;                    // batch index of the point in each slot
+    // re-sort scratch
+    int *sKeys = nullptr, *sVals = nullptr, *sKeys2 = nullptr, *sVals2 = nullptr, *sTmp = nullptr;
+    double *tpos = nullptr;
+    void *trest = nullptr;
+    unsigned char *tcrushed = nullptr;
+    int *torig = nullptr;
+    // checkpoint of the state
+    bool haveCheckpoint = false;
+    double *kpos = nullptr;
+    void *krest = nullptr;
+    unsigned char *kcrushed = nullptr;
+    int *korig = nullptr;
+    double kt = 0;
+    int kstep = 0;
+    std::vector<int> kcstart, kccase;
+    int kChunkPoints = 256;
+    double runMs = 0;   // device time of the last run's segments + re-sorts
 
     // device timing of the fused particle pass (stream launches only)
     int timingStride = 0;
@@ -883,11 +950,153 @@ hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const double *gV, do
     return hipGetLastError();
 }
 
+// Chunk table from the case-local cell keys of the points in slot order:
+// chunks of <= chunkPoints consecutive points of one case whose base cells
+// span at most two cell rows and 126 columns, so that a wavefront's window
+// (4 rows) holds all its stencils.  Two rows, not one: the ingest sorts by
+// the cell of the raw coordinates, and after the block offset the points of
+// one raw row can interleave two rows (the first raw row always does).
+void chunk_table(const icempm_batch *b, const std::vector<int> &keys, std::vector<int> &cstart,
+                 std::vector<int> &ccase) {
+    cstart.clear();
+    ccase.clear();
+    const int cap = b->chunkPoints;
+    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
+        const CaseParams &cp = b->hc[ci].cp;
+        const int n = cp.nPts, gx = cp.gx, o = cp.ptOffset;
+        if (n <= 0) continue;
+        int cs = 0;
+        cstart.push_back(o);
+        ccase.push_back((int)ci);
+        int r0 = keys[o] / gx, r1 = r0, c0 = keys[o] % gx, c1 = c0;
+        for (int i = 1; i < n; ++i) {
+            const int ki = keys[(size_t)o + i];
+            const int r = ki / gx, cc = ki % gx;
+            const int nr0 = std::min(r0, r), nr1 = std::max(r1, r), nc0 = std::min(c0, cc), nc1 = std::max(c1, cc);
+            const bool brk = (i - cs >= cap) || nr1 - nr0 > 1 || nc1 - nc0 > 125 || ki < 0 || keys[(size_t)o + cs] < 0;
+            if (brk) {
+                cstart.push_back(o + i);
+                ccase.push_back((int)ci);
+                cs = i;
+                r0 = r1 = r;
+                c0 = c1 = cc;
+            } else {
+                r0 = nr0; r1 = nr1; c0 = nc0; c1 = nc1;
+            }
+        }
+    }
+    cstart.push_back(b->nPts);
+}
+
+void drop_graph(icempm_batch *b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    b->graphSteps = 0;
+}
+
+// the host chunk table to the device (grows the device arrays when needed)
+int upload_chunks(icempm_batch *b) {
+    const size_t need = b->hcstart.size();
+    if (need > b->chunkCapDev) {
+        dfree(b->chunkStart);
+        dfree(b->chunkCase);
+        b->chunkCapDev = 0;
+        BCHK(b, hipMalloc(&b->chunkStart, need * sizeof(int)), "chunk table");
+        BCHK(b, hipMalloc(&b->chunkCase, need * sizeof(int)), "chunk table");
+        b->chunkCapDev = need;
+    }
+    BCHK(b, hipMemcpy(b->chunkStart, b->hcstart.data(), need * sizeof(int), hipMemcpyHostToDevice), "chunk table");
+    BCHK(b, hipMemcpy(b->chunkCase, b->hccase.data(), b->hccase.size() * sizeof(int), hipMemcpyHostToDevice),
+         "chunk table");
+    b->nChunks = (int)b->hccase.size();
+    drop_graph(b);
+    return ICEMPM_OK;
+}
+
+// the case-local cell keys of the current slots (and, for a sort, the
+// offset keys and slot indices in sKeys / sVals)
+int current_keys(icempm_batch *b, std::vector<int> &local) {
+    const int n = b->nPts;
+    if (!b->sKeys) {
+        BCHK(b, hipMalloc(&b->sKeys, (size_t)n * sizeof(int)), "keys");
+        BCHK(b, hipMalloc(&b->sVals, (size_t)n * sizeof(int)), "keys");
+    }
+    hipLaunchKernelGGL(k_batch_keys, dim3((n + 255) / 256), dim3(256), 0, b->stream, b->pos, b->pitch, n,
+                       b->cases, (int)b->hc.size(), b->sKeys, b->sVals);
+    BCHK(b, hipGetLastError(), "keys");
+    local.resize((size_t)n);
+    BCHK(b, hipMemcpyAsync(local.data(), b->sKeys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->stream), "keys");
+    BCHK(b, hipStreamSynchronize(b->stream), "keys");
+    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
+        const CaseParams &cp = b->hc[ci].cp;
+        for (int i = 0; i < cp.nPts; ++i) local[(size_t)cp.ptOffset + i] -= cp.gridOffset;
+    }
+    return ICEMPM_OK;
+}
+
+size_t rest_bytes(const icempm_batch *b) {
+    return 11 * b->pitch * (b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double));
+}
+
+// Stable re-sort of every case's points by their current cell, then a new
+// chunk table.  The slot -> batch-index map (orig) follows the points, so
+// reads return them in their original order.
+int resort(icempm_batch *b) {
+    const int n = b->nPts;
+    if (n <= 1) return ICEMPM_OK;
+    std::vector<int> local;
+    int rc = current_keys(b, local);
+    if (rc) return rc;
+    if (!b->sKeys2) {
+        BCHK(b, hipMalloc(&b->sKeys2, (size_t)n * sizeof(int)), "re-sort");
+        BCHK(b, hipMalloc(&b->sVals2, (size_t)n * sizeof(int)), "re-sort");
+        BCHK(b, hipMalloc(&b->sTmp, radix_tmp_ints((size_t)n) * sizeof(int)), "re-sort");
+        BCHK(b, hipMalloc(&b->tpos, 2 * b->pitch * sizeof(double)), "re-sort");
+        BCHK(b, hipMalloc(&b->trest, rest_bytes(b)), "re-sort");
+        BCHK(b, hipMalloc(&b->tcrushed, b->pitch), "re-sort");
+        BCHK(b, hipMalloc(&b->torig, b->pitch * sizeof(int)), "re-sort");
+    }
+    BCHK(b, radix_sort_pairs(b->sKeys, b->sVals, b->sKeys2, b->sVals2, (size_t)n, b->sTmp, b->stream), "re-sort");
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (b->precision == ICEMPM_BATCH_F32) {
+        Rows<true> a{b->pos, reinterpret_cast<float *>(b->rest), b->crushed, b->pitch};
+        Rows<true> t{b->tpos, reinterpret_cast<float *>(b->trest), b->tcrushed, b->pitch};
+        hipLaunchKernelGGL(k_batch_permute<true>, dim3(nb), dim3(256), 0, b->stream, a, t, b->orig, b->torig, b->sVals, n);
+    } else {
+        Rows<false> a{b->pos, reinterpret_cast<double *>(b->rest), b->crushed, b->pitch};
+        Rows<false> t{b->tpos, reinterpret_cast<double *>(b->trest), b->tcrushed, b->pitch};
+        hipLaunchKernelGGL(k_batch_permute<false>, dim3(nb), dim3(256), 0, b->stream, a, t, b->orig, b->torig, b->sVals, n);
+    }
+    BCHK(b, hipGetLastError(), "re-sort");
+    std::swap(b->pos, b->tpos);
+    std::swap(b->rest, b->trest);
+    std::swap(b->crushed, b->tcrushed);
+    std::swap(b->orig, b->torig);
+    // the sorted keys give the new chunk table
+    BCHK(b, hipMemcpyAsync(local.data(), b->sKeys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->stream), "re-sort");
+    BCHK(b, hipStreamSynchronize(b->stream), "re-sort");
+    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
+        const CaseParams &cp = b->hc[ci].cp;
+        for (int i = 0; i < cp.nPts; ++i) local[(size_t)cp.ptOffset + i] -= cp.gridOffset;
+    }
+    chunk_table(b, local, b->hcstart, b->hccase);
+    return upload_chunks(b);
+}
+
+// the chunk table for the current slots and chunkPoints
+int rebuild_chunks(icempm_batch *b) {
+    std::vector<int> local;
+    int rc = current_keys(b, local);
+    if (rc) return rc;
+    chunk_table(b, local, b->hcstart, b->hccase);
+    return upload_chunks(b);
+}
+
 // The substep loop of n substeps starting from grid A, as stream work.
-hipError_t enqueue_loop(icempm_batch *b, int n) {
+hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist) {
     hipError_t e = launch_fill_f64(b->gridA, b->gpitch * 3, 0.0, b->stream);
     if (e != hipSuccess) return e;
-    if ((e = launch_fill_f64(b->hist, 2 * (size_t)n * b->hc.size(), 0.0, b->stream)) != hipSuccess) return e;
+    if ((e = launch_fill_f64(hist, 2 * (size_t)n * b->hc.size(), 0.0, b->stream)) != hipSuccess) return e;
     if (b->mapCount &&
         (e = launch_fill_i32(reinterpret_cast<int *>(b->mapCount), 2, 0, b->stream)) != hipSuccess)
         return e;
@@ -897,8 +1106,8 @@ hipError_t enqueue_loop(icempm_batch *b, int n) {
     for (int i = 0; i < n; ++i) {
         const bool last = (i == n - 1);
         hipLaunchKernelGGL(k_batch_update, dim3(b->nNodeBlocks), dim3(256), 0, b->stream, b->cases,
-                           b->nodeBlock, A, last ? nullptr : B, b->gpitch, b->tTab, i,
-                           b->hist + 2 * nc * (size_t)i);
+                           b->nodeBlock, A, last ? nullptr : B, b->gpitch, tTab, i,
+                           hist + 2 * nc * (size_t)i);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (!last) {
             const bool timeIt = b->timingStride > 0 && i % b->timingStride == 0;
@@ -956,6 +1165,9 @@ void icempm_batch_destroy(icempm_batch *b) {
     dfree(b->err); dfree(b->tTab); dfree(b->hist);
     dfree(b->mlpW); dfree(b->mlpP); dfree(b->hx); dfree(b->ht); dfree(b->hcount); dfree(b->stats);
     dfree(b->mapQ); dfree(b->mapCount);
+    dfree(b->orig); dfree(b->sKeys); dfree(b->sVals); dfree(b->sKeys2); dfree(b->sVals2); dfree(b->sTmp);
+    dfree(b->tpos); dfree(b->trest); dfree(b->tcrushed); dfree(b->torig);
+    dfree(b->kpos); dfree(b->krest); dfree(b->kcrushed); dfree(b->korig);
     for (hipEvent_t ev : b->tev) (void)hipEventDestroy(ev);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
@@ -1017,59 +1229,34 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
 int icempm_batch_finalize(icempm_batch *b) {
     if (!b || b->hc.empty()) return ICEMPM_E_ARG;
     if (b->finalized) return bfail(b, ICEMPM_E_STATE, "finalize: already finalized");
-    // point and node offsets, chunk and node-block tables
-    std::vector<int> cstart, ccase;
+    // point and node offsets, node-block table, keys of the points in slot order
     std::vector<int2> nblk;
+    std::vector<int> keys;
     long pofs = 0, gofs = 0;
     for (size_t ci = 0; ci < b->hc.size(); ++ci) {
         auto &h = b->hc[ci];
         h.cp.ptOffset = (int)pofs;
         h.cp.gridOffset = (int)gofs;
-        const int n = h.cp.nPts, gx = h.cp.gx;
-        // chunks of <= 256 consecutive points whose base cells span at most two
-        // cell rows and 126 columns: a wavefront's window (4 rows) then holds
-        // all its stencils.  Two rows, not one: the ingest sorts by the cell of
-        // the raw coordinates, and after the block offset the points of one raw
-        // row can interleave two rows (the first raw row always does).
-        int cs = 0;
-        cstart.push_back((int)pofs);
-        ccase.push_back((int)ci);
-        int r0 = h.keys[0] / gx, r1 = r0, c0 = h.keys[0] % gx, c1 = c0;
-        for (int i = 1; i < n; ++i) {
-            const int ki = h.keys[i];
-            const int r = ki / gx, cc = ki % gx;
-            const int nr0 = std::min(r0, r), nr1 = std::max(r1, r), nc0 = std::min(c0, cc), nc1 = std::max(c1, cc);
-            const bool brk = (i - cs >= 256) || nr1 - nr0 > 1 || nc1 - nc0 > 125 || ki < 0 || h.keys[cs] < 0;
-            if (brk) {
-                cstart.push_back((int)(pofs + i));
-                ccase.push_back((int)ci);
-                cs = i;
-                r0 = r1 = r;
-                c0 = c1 = cc;
-            } else {
-                r0 = nr0; r1 = nr1; c0 = nc0; c1 = nc1;
-            }
-        }
+        keys.insert(keys.end(), h.keys.begin(), h.keys.end());
         for (int nd = 0; nd < h.cp.gridTotal; nd += 256) nblk.push_back(make_int2((int)ci, nd));
-        pofs += n;
+        pofs += h.cp.nPts;
         gofs += h.cp.gridTotal;
     }
-    cstart.push_back((int)pofs);
+    if (pofs > INT_MAX / 2 || gofs > INT_MAX / 2) return bfail(b, ICEMPM_E_ARG, "finalize: batch too large");
     b->nPts = (int)pofs;
     b->nNodes = gofs;
-    b->nChunks = (int)ccase.size();
     b->nNodeBlocks = (int)nblk.size();
     b->pitch = (pofs + 63) / 64 * 64;
     b->gpitch = ((size_t)gofs + 63) / 64 * 64;
+    chunk_table(b, keys, b->hcstart, b->hccase);
     const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
     BCHK(b, hipMalloc(&b->pos, 2 * b->pitch * sizeof(double)), "batch finalize");
     BCHK(b, hipMalloc(&b->rest, 11 * b->pitch * vs), "batch finalize");
     BCHK(b, hipMalloc(&b->crushed, b->pitch), "batch finalize");
+    BCHK(b, hipMalloc(&b->orig, b->pitch * sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->gridA, 3 * b->gpitch * sizeof(double)), "batch finalize");
     BCHK(b, hipMalloc(&b->gridB, 3 * b->gpitch * sizeof(double)), "batch finalize");
     BCHK(b, hipMalloc(&b->cases, b->hc.size() * sizeof(CaseParams)), "batch finalize");
-    BCHK(b, hipMalloc(&b->chunkStart, cstart.size() * sizeof(int)), "batch finalize");
-    BCHK(b, hipMalloc(&b->chunkCase, ccase.size() * sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->nodeBlock, nblk.size() * sizeof(int2)), "batch finalize");
     BCHK(b, hipMalloc(&b->err, sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->hcount, sizeof(unsigned long long)), "batch finalize");
@@ -1077,6 +1264,7 @@ int icempm_batch_finalize(icempm_batch *b) {
     // host staging in the device layout
     std::vector<double> hpos(2 * b->pitch, 0.0);
     std::vector<unsigned char> hrest(11 * b->pitch * vs, 0), hcr(b->pitch, 0);
+    std::vector<int> horig(b->pitch, 0);
     std::vector<CaseParams> hcp;
     for (auto &h : b->hc) {
         const int n = h.cp.nPts, o = h.cp.ptOffset;
@@ -1084,6 +1272,7 @@ int icempm_batch_finalize(icempm_batch *b) {
             hpos[o + i] = h.pos[i];
             hpos[b->pitch + o + i] = h.pos[n + i];
             hcr[o + i] = h.crushed[i];
+            horig[o + i] = o + i;
             for (int r = 0; r < 11; ++r) {
                 double v = h.rest[(size_t)r * n + i];
                 if (b->precision == ICEMPM_BATCH_F32) {
@@ -1100,29 +1289,77 @@ int icempm_batch_finalize(icempm_batch *b) {
     BCHK(b, hipMemcpy(b->pos, hpos.data(), hpos.size() * sizeof(double), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->rest, hrest.data(), hrest.size(), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->crushed, hcr.data(), hcr.size(), hipMemcpyHostToDevice), "batch upload");
+    BCHK(b, hipMemcpy(b->orig, horig.data(), horig.size() * sizeof(int), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->cases, hcp.data(), hcp.size() * sizeof(CaseParams), hipMemcpyHostToDevice), "batch upload");
-    BCHK(b, hipMemcpy(b->chunkStart, cstart.data(), cstart.size() * sizeof(int), hipMemcpyHostToDevice), "batch upload");
-    BCHK(b, hipMemcpy(b->chunkCase, ccase.data(), ccase.size() * sizeof(int), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->nodeBlock, nblk.data(), nblk.size() * sizeof(int2), hipMemcpyHostToDevice), "batch upload");
     int zero = 0;
     BCHK(b, hipMemcpy(b->err, &zero, sizeof(int), hipMemcpyHostToDevice), "batch upload");
     const unsigned long long z3[3] = {0, 0, 0};
     BCHK(b, hipMemcpy(b->hcount, z3, sizeof(unsigned long long), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->stats, z3, sizeof(z3), hipMemcpyHostToDevice), "batch upload");
+    int rc = upload_chunks(b);
+    if (rc) return rc;
     b->finalized = true;
     return ICEMPM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// n substeps from grid A with the given time table and force-history rows,
+// as one graph launch (captured when the step count or the pointers change)
+// or as stream launches; returns the device time in ms.
+int run_segment(icempm_batch *b, int n, int use_graph, const double *tTab, double *hist, float *ms) {
+    b->tevNext = 0;
+    if (use_graph) {
+        if (!b->graph || b->graphSteps != n || b->graphT != tTab || b->graphH != hist) {
+            drop_graph(b);
+            hipGraph_t g = nullptr;
+            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch capture");
+            hipError_t e = enqueue_loop(b, n, tTab, hist);
+            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess) return bfail(b, ICEMPM_E_HIP, std::string("batch capture: ") + hipGetErrorString(e));
+            BCHK(b, e2, "batch capture");
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            BCHK(b, e, "batch graph instantiate");
+            b->graphSteps = n;
+            b->graphT = tTab;
+            b->graphH = hist;
+        }
+        // timed from here: capture and instantiation are host work before it
+        BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
+        BCHK(b, hipGraphLaunch(b->graph, b->stream), "batch graph launch");
+    } else {
+        BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
+        BCHK(b, enqueue_loop(b, n, tTab, hist), "batch run");
+    }
+    BCHK(b, hipEventRecord(b->e1, b->stream), "batch run");
+    BCHK(b, hipEventSynchronize(b->e1), "batch run");
+    BCHK(b, hipEventElapsedTime(ms, b->e0, b->e1), "batch run");
+    for (size_t q = 0; q + 1 < b->tevNext; q += 2) {
+        float kms = 0;
+        BCHK(b, hipEventElapsedTime(&kms, b->tev[q], b->tev[q + 1]), "batch timing");
+        b->timedMs += kms;
+        b->timedLaunches += 1;
+    }
+    b->tevNext = 0;
+    return ICEMPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int icempm_batch_run(icempm_batch *b, int n, int use_graph) {
     if (!b || n <= 0) return ICEMPM_E_ARG;
     if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "run: batch not finalized");
+    if (use_graph && b->timingStride > 0)
+        return bfail(b, ICEMPM_E_STATE, "run: kernel timing needs stream launches (use_graph = 0)");
     const size_t nc = b->hc.size();
     if (n > b->cap) {
-        if (b->graph) {
-            (void)hipGraphExecDestroy(b->graph);
-            b->graph = nullptr;
-            b->graphSteps = 0;
-        }
+        drop_graph(b);
         dfree(b->tTab);
         dfree(b->hist);
         BCHK(b, hipMalloc(&b->tTab, (size_t)n * sizeof(double)), "batch run");
@@ -1138,41 +1375,26 @@ int icempm_batch_run(icempm_batch *b, int n, int use_graph) {
     }
     BCHK(b, hipMemcpyAsync(b->tTab, tt.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream), "batch run");
     BCHK(b, hipStreamSynchronize(b->stream), "batch run");
-    if (use_graph && b->timingStride > 0)
-        return bfail(b, ICEMPM_E_STATE, "run: kernel timing needs stream launches (use_graph = 0)");
-    b->tevNext = 0;
-    if (use_graph) {
-        if (!b->graph || b->graphSteps != n) {
-            if (b->graph) (void)hipGraphExecDestroy(b->graph);
-            b->graph = nullptr;
-            hipGraph_t g = nullptr;
-            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch capture");
-            hipError_t e = enqueue_loop(b, n);
-            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-            if (e != hipSuccess) return bfail(b, ICEMPM_E_HIP, std::string("batch capture: ") + hipGetErrorString(e));
-            BCHK(b, e2, "batch capture");
-            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            BCHK(b, e, "batch graph instantiate");
-            b->graphSteps = n;
+    // segments of resortEvery substeps, the points re-sorted between them
+    const int seg = b->resortEvery > 0 ? b->resortEvery : n;
+    double total = 0;
+    for (int s0 = 0; s0 < n; s0 += seg) {
+        if (s0 > 0) {
+            const auto h0 = std::chrono::steady_clock::now();
+            int rc = resort(b);
+            if (rc) return rc;
+            total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
         }
-        // timed from here: capture and instantiation are host work before it
-        BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
-        BCHK(b, hipGraphLaunch(b->graph, b->stream), "batch graph launch");
-    } else {
-        BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
-        BCHK(b, enqueue_loop(b, n), "batch run");
-    }
-    BCHK(b, hipEventRecord(b->e1, b->stream), "batch run");
-    BCHK(b, hipEventSynchronize(b->e1), "batch run");
-    BCHK(b, hipEventElapsedTime(&b->lastMs, b->e0, b->e1), "batch run");
-    for (size_t q = 0; q + 1 < b->tevNext; q += 2) {
+        const int len = std::min(seg, n - s0);
         float ms = 0;
-        BCHK(b, hipEventElapsedTime(&ms, b->tev[q], b->tev[q + 1]), "batch timing");
-        b->timedMs += ms;
-        b->timedLaunches += 1;
+        const auto h0 = std::chrono::steady_clock::now();
+        int rc = run_segment(b, len, use_graph, b->tTab + s0, b->hist + 2 * nc * (size_t)s0, &ms);
+        if (rc) return rc;
+        // with re-sorting, each segment is captured anew: count the host time
+        total += b->resortEvery > 0 ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count()
+                                    : (double)ms;
     }
-    b->tevNext = 0;
+    b->lastMs = (float)total;
     b->t = t;
     b->step += n;
     b->lastRun = n;
@@ -1231,21 +1453,26 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
         BCHK(b, hipMemcpy(hrest.data() + (size_t)r * n * vs, (unsigned char *)b->rest + ((size_t)r * b->pitch + o) * vs,
                           n * vs, hipMemcpyDeviceToHost), "read_points");
     BCHK(b, hipMemcpy(hcr.data(), b->crushed + o, n, hipMemcpyDeviceToHost), "read_points");
+    std::vector<int> horig((size_t)n);
+    BCHK(b, hipMemcpy(horig.data(), b->orig + o, n * sizeof(int), hipMemcpyDeviceToHost), "read_points");
     std::memset(soa, 0, 16 * pitch * sizeof(double));
     unsigned char *u8 = reinterpret_cast<unsigned char *>(soa);
-    for (int i = 0; i < n; ++i) {
-        u8[i] = hcr[i];
-        soa[pitch * 4 + i] = hpos[i];
-        soa[pitch * 5 + i] = hpos[n + i];
+    for (int slot = 0; slot < n; ++slot) {
+        // the point in this slot, at its index within the case
+        const int i = horig[slot] - o;
+        if (i < 0 || i >= n) return bfail(b, ICEMPM_E_STATE, "read_points: slot map corrupt");
+        u8[i] = hcr[slot];
+        soa[pitch * 4 + i] = hpos[slot];
+        soa[pitch * 5 + i] = hpos[n + slot];
         for (int r = 0; r < 11; ++r) {
             double v;
             if (vs == sizeof(float)) {
                 float f;
-                std::memcpy(&f, &hrest[((size_t)r * n + i) * vs], vs);
+                std::memcpy(&f, &hrest[((size_t)r * n + slot) * vs], vs);
                 v = f;
                 if (r == 2 || r == 5) v += 1.0;
             } else {
-                std::memcpy(&v, &hrest[((size_t)r * n + i) * vs], vs);
+                std::memcpy(&v, &hrest[((size_t)r * n + slot) * vs], vs);
             }
             if (r < 10)
                 soa[pitch * (6 + r) + i] = v;
@@ -1329,6 +1556,126 @@ int icempm_batch_read_harvest(icempm_batch *b, float *X, float *T, int cap, unsi
         BCHK(b, hipMemcpy(T, b->ht, (size_t)m * 3 * sizeof(float), hipMemcpyDeviceToHost), "read_harvest");
     }
     return ICEMPM_OK;
+}
+
+int icempm_batch_set_tuning(icempm_batch *b, int knob, int value) {
+    if (!b) return ICEMPM_E_ARG;
+    switch (knob) {
+    case ICEMPM_BATCH_TUNE_CHUNK:
+        if (value != 64 && value != 128 && value != 256)
+            return bfail(b, ICEMPM_E_ARG, "set_tuning: points per chunk is 64, 128 or 256");
+        if (value != b->chunkPoints) {
+            b->chunkPoints = value;
+            if (b->finalized) {
+                BCHK(b, hipStreamSynchronize(b->stream), "set_tuning");
+                return rebuild_chunks(b);
+            }
+        }
+        return ICEMPM_OK;
+    case ICEMPM_BATCH_TUNE_RESORT:
+        if (value < 0) return bfail(b, ICEMPM_E_ARG, "set_tuning: the re-sort cadence is >= 0");
+        b->resortEvery = value;
+        return ICEMPM_OK;
+    default:
+        return bfail(b, ICEMPM_E_ARG, "set_tuning: unknown knob");
+    }
+}
+
+int icempm_batch_get_tuning(const icempm_batch *b, int knob) {
+    if (!b) return -1;
+    switch (knob) {
+    case ICEMPM_BATCH_TUNE_CHUNK: return b->chunkPoints;
+    case ICEMPM_BATCH_TUNE_RESORT: return b->resortEvery;
+    default: return -1;
+    }
+}
+
+int icempm_batch_resort(icempm_batch *b) {
+    if (!b) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "resort: batch not finalized");
+    BCHK(b, hipStreamSynchronize(b->stream), "resort");
+    return resort(b);
+}
+
+int icempm_batch_checkpoint(icempm_batch *b) {
+    if (!b) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "checkpoint: batch not finalized");
+    if (!b->kpos) {
+        BCHK(b, hipMalloc(&b->kpos, 2 * b->pitch * sizeof(double)), "checkpoint");
+        BCHK(b, hipMalloc(&b->krest, rest_bytes(b)), "checkpoint");
+        BCHK(b, hipMalloc(&b->kcrushed, b->pitch), "checkpoint");
+        BCHK(b, hipMalloc(&b->korig, b->pitch * sizeof(int)), "checkpoint");
+    }
+    BCHK(b, hipMemcpyAsync(b->kpos, b->pos, 2 * b->pitch * sizeof(double), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+    BCHK(b, hipMemcpyAsync(b->krest, b->rest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+    BCHK(b, hipMemcpyAsync(b->kcrushed, b->crushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+    BCHK(b, hipMemcpyAsync(b->korig, b->orig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+    BCHK(b, hipStreamSynchronize(b->stream), "checkpoint");
+    b->kt = b->t;
+    b->kstep = b->step;
+    b->kcstart = b->hcstart;
+    b->kccase = b->hccase;
+    b->kChunkPoints = b->chunkPoints;
+    b->haveCheckpoint = true;
+    return ICEMPM_OK;
+}
+
+int icempm_batch_restore(icempm_batch *b) {
+    if (!b) return ICEMPM_E_ARG;
+    if (!b->haveCheckpoint) return bfail(b, ICEMPM_E_STATE, "restore: no checkpoint");
+    BCHK(b, hipStreamSynchronize(b->stream), "restore");
+    BCHK(b, hipMemcpyAsync(b->pos, b->kpos, 2 * b->pitch * sizeof(double), hipMemcpyDeviceToDevice, b->stream), "restore");
+    BCHK(b, hipMemcpyAsync(b->rest, b->krest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "restore");
+    BCHK(b, hipMemcpyAsync(b->crushed, b->kcrushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "restore");
+    BCHK(b, hipMemcpyAsync(b->orig, b->korig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "restore");
+    BCHK(b, hipStreamSynchronize(b->stream), "restore");
+    b->t = b->kt;
+    b->step = b->kstep;
+    b->hcstart = b->kcstart;
+    b->hccase = b->kccase;
+    b->chunkPoints = b->kChunkPoints;
+    return upload_chunks(b);
+}
+
+int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int cap, int *rows) {
+    if (!b || probe_steps < 16 || cap < 0 || (cap > 0 && !report) || !rows) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "autotune: batch not finalized");
+    if (b->timingStride > 0) return bfail(b, ICEMPM_E_STATE, "autotune: switch kernel timing off first");
+    int rc = icempm_batch_checkpoint(b);
+    if (rc) return rc;
+    const int chunks[3] = {256, 128, 64};
+    const int resorts[3] = {0, probe_steps / 2, probe_steps / 4};
+    double best = 1e300;
+    int bestChunk = b->chunkPoints, bestResort = b->resortEvery, nrow = 0;
+    for (int ci = 0; ci < 3; ++ci)
+        for (int ri = 0; ri < 3; ++ri) {
+            // the run's time: device time of the graph launch without
+            // re-sorting; with re-sorting, the host time of every segment
+            // (re-sort, capture of the new chunk table, launch)
+            if ((rc = icempm_batch_restore(b))) return rc;
+            b->chunkPoints = chunks[ci];
+            b->resortEvery = resorts[ri];
+            if ((rc = rebuild_chunks(b))) return rc;
+            if ((rc = icempm_batch_run(b, probe_steps, 1))) return rc;
+            const double per = b->lastMs / probe_steps;
+            if (nrow < cap) {
+                report[3 * nrow] = chunks[ci];
+                report[3 * nrow + 1] = resorts[ri];
+                report[3 * nrow + 2] = per;
+            }
+            ++nrow;
+            if (per < best) {
+                best = per;
+                bestChunk = chunks[ci];
+                bestResort = resorts[ri];
+            }
+        }
+    *rows = nrow;
+    // back to the checkpointed state, with the fastest setting
+    if ((rc = icempm_batch_restore(b))) return rc;
+    b->chunkPoints = bestChunk;
+    b->resortEvery = bestResort;
+    return rebuild_chunks(b);
 }
 
 int icempm_batch_map_stats(icempm_batch *b, unsigned long long *out, int reset) {

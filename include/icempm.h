@@ -230,6 +230,27 @@ int icempm_batch_force_history(icempm_batch *b, int c, double *out, int n);
 /* the state of case c in the reference SoA layout (P, Q and grain zero) */
 int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch);
 int icempm_batch_error_flag(icempm_batch *b, int *flag);
+/* Tuning knobs: ICEMPM_BATCH_TUNE_CHUNK, points per chunk = threads per
+ * workgroup of the particle pass (64, 128 or 256; default 256);
+ * ICEMPM_BATCH_TUNE_RESORT, re-sort the points by cell every that many
+ * substeps within a run (0, the default: never).  get returns -1 for an
+ * unknown knob. */
+enum { ICEMPM_BATCH_TUNE_CHUNK = 0, ICEMPM_BATCH_TUNE_RESORT = 1 };
+int icempm_batch_set_tuning(icempm_batch *b, int knob, int value);
+int icempm_batch_get_tuning(const icempm_batch *b, int knob);
+/* stable re-sort of every case's points by current cell (reads keep returning
+ * the points in their original order) and a new chunk table */
+int icempm_batch_resort(icempm_batch *b);
+/* device copy of the point state, time and step (one checkpoint per batch);
+ * restore returns to it */
+int icempm_batch_checkpoint(icempm_batch *b);
+int icempm_batch_restore(icempm_batch *b);
+/* Autotuner: from a checkpoint of the current state, times probe_steps
+ * substeps (graph replay) for every points-per-chunk value and the re-sort
+ * cadences {never, probe/2, probe/4}; report receives rows (chunk, re-sort,
+ * ms per substep), *rows their count; the state is restored and the fastest
+ * setting is left in place. */
+int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int cap, int *rows);
 /* Return map of crushed points (gpu_implementation5.cu:48-94 is the exact
  * one).  ICEMPM_MAP_EXACT: the exact FP64 map.  ICEMPM_MAP_HARVEST: the
  * exact map, and every crushed point with positive singular values also

@@ -66,3 +66,41 @@ def test_batch_graph_replay_matches_stream_run(gpu_available):
         assert np.abs(g.points(c)[4:6, :n] - s.points(c)[4:6, :n]).max() <= 1e-12
     g.close()
     s.close()
+
+
+def test_resort_chunk_sizes_and_checkpoint_match_oracle(gpu_available):
+    """Re-sorting between segments, other points-per-chunk values and a
+    checkpoint/restore leave every case on the oracle's trajectory, and the
+    points read back in their original order."""
+    print("batch resort/chunk/checkpoint", flush=True)
+    cases = batch_cases(3, 2000, 6000, seed=13)
+    nsub = 120
+    b = build_batch(cases, BATCH_F64)
+    refs = [oracle_case(*b.initial[c], nsub) for c in range(len(cases))]
+
+    def check(label):
+        assert b.error_flag() == 0
+        for c, case in enumerate(cases):
+            n = case.n_points
+            href, pref = refs[c]
+            assert rel_l2(b.force_history(c, nsub), href) <= 1e-6, (label, c)
+            assert np.abs(b.points(c)[4:6, :n] - pref[4:6, :n]).max() <= 1e-12, (label, c)
+
+    b.checkpoint()
+    b.set_tuning(chunk=64, resort_every=50)      # segments 50, 50, 20 with re-sorts between
+    assert b.tuning() == (64, 50)
+    b.run(nsub)
+    check("chunk 64, re-sort every 50")
+    b.restore()
+    b.set_tuning(chunk=128, resort_every=0)
+    b.resort()
+    b.run(nsub, use_graph=False)
+    check("chunk 128, re-sorted once")
+    b.restore()
+    rows = b.autotune(probe_steps=40)
+    print("autotune", rows, "->", b.tuning(), flush=True)
+    assert len(rows) == 9 and all(r[2] > 0 for r in rows)
+    assert b.tuning() in {(r[0], r[1]) for r in rows}
+    b.run(nsub)                                   # the state was restored by the tuner
+    check("after autotune")
+    b.close()
