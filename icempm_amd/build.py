@@ -5,6 +5,7 @@
                      GPU_Implementation5) over the C ABI; plain g++
   libicempm_h5.so    SnapshotManager's HDF5 files (when the HDF5 C library is
                      present); plain g++
+  cm                 the command-line frontend over the three (with HDF5)
 
 Run ``python -m icempm_amd.build`` (or ``__graft_entry__.build()``).  Both
 libraries are written next to this file so that they travel with the tree.
@@ -108,6 +109,22 @@ def build_h5() -> str | None:
     return H5_LIB
 
 
+CLI_SRC = os.path.join(HERE, "cli", "cm.cpp")
+CLI_BIN = os.path.join(HERE, "cm")
+
+
+def build_cli() -> str | None:
+    """The cm frontend (cli/main.cpp's behaviour), when libicempm_h5.so is built."""
+    if not os.path.exists(H5_LIB):
+        return None
+    cxx = os.environ.get("CXX", "g++")
+    tmp = CLI_BIN + ".tmp"
+    _run([cxx, *HOST_FLAGS, "-o", tmp, CLI_SRC, f"-L{HERE}", "-licempm_h5", "-licempm_host",
+          "-licempm_hip", "-Wl,-rpath,$ORIGIN", "-pthread"])
+    os.replace(tmp, CLI_BIN)
+    return CLI_BIN
+
+
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "drop_in_driver.cpp")
 DRIVER_BIN = os.path.join(ROOT, "tests", "cpp", "drop_in_driver")
 
@@ -127,6 +144,7 @@ def build_all() -> list[str]:
     h5 = build_h5()
     if h5:
         out.append(h5)
+        out.append(build_cli())
     if os.path.exists(DRIVER_SRC):
         out.append(build_driver())
     return out
@@ -134,7 +152,7 @@ def build_all() -> list[str]:
 
 def clean() -> None:
     shutil.rmtree(OBJ, ignore_errors=True)
-    for p in (HIP_LIB, HOST_LIB, H5_LIB, DRIVER_BIN):
+    for p in (HIP_LIB, HOST_LIB, H5_LIB, CLI_BIN, DRIVER_BIN):
         if os.path.exists(p):
             os.remove(p)
 
