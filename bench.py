@@ -56,19 +56,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(name: str):
-    """HBM bytes per launch of the fused kernel, from the committed rocprofv3
+def profile_value(name: str, key: str):
+    """A per-launch figure of the fused kernel from the committed rocprofv3
     counter summary (profiles/<name>_counters.txt), if any."""
     path = os.path.join(ROOT, "profiles", f"{name}_counters.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("hbm_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             try:
                 return float(line.split(":", 1)[1].split()[0])
             except ValueError:
                 return None
     return None
+
+
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch of the fused kernel (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    return profile_value(name, "hbm_bytes_per_launch")
+
+
+def atomics_from_profiles(name: str, avg_launch_s: float):
+    """Global atomic requests at L2 per launch (TCC_ATOMIC_sum) and their rate
+    at the measured launch duration."""
+    n = profile_value(name, "tcc_atomic_requests_per_launch")
+    if n is None or not avg_launch_s:
+        return None
+    return {"l2_atomic_requests_per_launch": n, "l2_atomic_requests_per_s": n / avg_launch_s}
 
 
 def cpu_baseline(case, pts0, prms0, target_s: float, threads: int):
@@ -194,6 +208,7 @@ def run_single(args) -> dict:
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic_from_profiles(f"fused_g2p_p2g_{case.name}"),
+            "atomics": atomics_from_profiles(f"fused_g2p_p2g_{case.name}", k_avg_s),
             "bytes_per_launch": k_bytes,
             "avg_launch_us": k_avg_s * 1e6,
             "timed_launches": klaunch,
@@ -362,6 +377,7 @@ def run_batch_learned(args) -> dict:
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic_from_profiles("batch_particles_learned_32cases"),
+            "atomics": atomics_from_profiles("batch_particles_learned_32cases", k_avg_s),
             "bytes_per_launch": k_bytes,
             "avg_launch_us": k_avg_s * 1e6,
             "timed_launches": klaunch,

@@ -581,13 +581,15 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
                                                   double *__restrict__ gridOut, size_t gpitch,
                                                   int *__restrict__ err, LearnArgs la) {
     __shared__ mlpdev::InferScratch isc[4];
-    __shared__ unsigned short wsh[mlpdev::kParams];   // the weights, BF16, staged once per block
+    __shared__ __attribute__((aligned(16))) unsigned short wsh[mlpdev::kParams];   // the weights, BF16
     using V = typename Rows<F32>::V;
     constexpr double fe0 = F32 ? 1.0 : 0.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) qCountNext[0] = 0u;
     const unsigned int n = qCount[0];
     if (blockIdx.x * 64u >= n) return;   // no entry for this block (uniform)
-    for (int x = threadIdx.x; x < mlpdev::kParams; x += 256) wsh[x] = la.wbf[x];
+    static_assert(mlpdev::kParams % 8 == 0, "weights are staged 8 at a time");
+    for (int x = threadIdx.x; x < mlpdev::kParams / 8; x += 256)
+        reinterpret_cast<uint4 *>(wsh)[x] = reinterpret_cast<const uint4 *>(la.wbf)[x];
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const size_t pp = S.pitch;

@@ -276,14 +276,24 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ para
 // grad = sum over the partial vectors; then one Adam step on the FP32
 // master parameters.  Parameters that are padding stay zero.
 // The BF16 copy used by inference is refreshed from the result.
-__global__ void k_mlp_adam(float *__restrict__ params, unsigned short *__restrict__ wbf,
-                           float *__restrict__ mom1, float *__restrict__ mom2,
-                           const float *__restrict__ part, int nparts, float *__restrict__ grad,
-                           float lr, float b1, float b2, float eps, float c1, float c2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kParams) return;
-    float gsum = 0;
-    for (int p = 0; p < nparts; ++p) gsum += part[(size_t)p * kParams + i];
+// A workgroup of 256 handles 64 parameters: its four wavefronts each sum a
+// quarter of the partial vectors (64 consecutive floats per load), then the
+// first wavefront adds the four sums and takes the step.
+__global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ params, unsigned short *__restrict__ wbf,
+                                                  float *__restrict__ mom1, float *__restrict__ mom2,
+                                                  const float *__restrict__ part, int nparts,
+                                                  float *__restrict__ grad, float lr, float b1, float b2,
+                                                  float eps, float c1, float c2) {
+    __shared__ float qs[4][64];
+    const int l = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + l;
+    float g4 = 0;
+    if (i < kParams)
+        for (int p = q; p < nparts; p += 4) g4 += part[(size_t)p * kParams + i];
+    qs[q][l] = g4;
+    __syncthreads();
+    if (q != 0 || i >= kParams) return;
+    const float gsum = (qs[0][l] + qs[1][l]) + (qs[2][l] + qs[3][l]);
     grad[i] = gsum;
     // padding: input columns >= 5 of W1, output rows >= 3 of W3 and b3
     bool pad = false;
@@ -400,7 +410,7 @@ int icempm_mlp_step(icempm_mlp *m, const float *X, const float *T, int r0, int B
     if (lr > 0) m->t += 1;
     const float c1 = 1 - std::pow(b1, (float)(m->t > 0 ? m->t : 1));
     const float c2 = 1 - std::pow(b2, (float)(m->t > 0 ? m->t : 1));
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, m->stream, m->params, m->wbf,
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 63) / 64), dim3(256), 0, m->stream, m->params, m->wbf,
                        m->m1, m->m2, m->part, m->nparts, m->grad, lr, b1, b2, 1e-8f, c1, c2);
     MCHK(m, hipGetLastError(), "mlp adam");
     if (grad_out)
