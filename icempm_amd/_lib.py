@@ -152,6 +152,7 @@ HOST_EXPORTS = [
 
 H5_EXPORTS = [
     "icempm_model_load_raw_points_file", "icempm_model_save_snapshot", "icempm_model_save_pq",
+    "icempm_model_read_snapshot",
     "icempm_h5_write_raw_points",
 ]
 
@@ -296,6 +297,7 @@ def h5():
     _sig(L, "icempm_model_load_raw_points_file", c_int, P, c_char_p)
     _sig(L, "icempm_model_save_snapshot", c_int, P, c_char_p)
     _sig(L, "icempm_model_save_pq", c_int, P, c_char_p)
+    _sig(L, "icempm_model_read_snapshot", c_int, P, c_char_p)
     _sig(L, "icempm_h5_write_raw_points", c_int, c_char_p, POINTER(c_float), POINTER(c_int16), c_int,
          c_float, c_int)
     _h5 = L

@@ -171,7 +171,67 @@ void icy::SnapshotManager::LoadRawPoints(std::string fileName)
     LoadRawPoints(xy.data(), grains.data(), n, volume);
 }
 
+// Resume from a snapshot (the reference's ReadSnapshot, snapshotmanager.cpp:147-184,
+// is commented out there): the parameter struct from Params (ParticleViewSize
+// kept), arrays allocated for its grid and point count, the point SoA from
+// Points (re-pitched if the file's pitch differs), uploaded, constants updated.
+void icy::SnapshotManager::ReadSnapshot(std::string fileName)
+{
+    if (!model) throw std::runtime_error("SnapshotManager: no model");
+    if (!std::filesystem::exists(fileName)) throw std::runtime_error("ReadSnapshot: no file");
+    H file(check_id(H5Fopen(fileName.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT), fileName.c_str()), H5Fclose);
+    H dp(check_id(H5Dopen2(file, "Params", H5P_DEFAULT), "Params"), H5Dclose);
+    {
+        H sp(check_id(H5Dget_space(dp), "Params space"), H5Sclose);
+        hsize_t n = 0;
+        if (H5Sget_simple_extent_ndims(sp) != 1) throw std::runtime_error("ReadSnapshot: SimParams size mismatch");
+        H5Sget_simple_extent_dims(sp, &n, nullptr);
+        if (n != sizeof(icy::SimParams)) throw std::runtime_error("ReadSnapshot: SimParams size mismatch");
+    }
+    icy::SimParams tmp;
+    check(H5Dread(dp, H5T_NATIVE_B8, H5S_ALL, H5S_ALL, H5P_DEFAULT, &tmp), "reading Params");
+    H dpts(check_id(H5Dopen2(file, "Points", H5P_DEFAULT), "Points"), H5Dclose);
+    const size_t filePitch = (size_t)tmp.nPtsPitch;
+    {
+        H sp(check_id(H5Dget_space(dpts), "Points space"), H5Sclose);
+        hsize_t n = 0;
+        H5Sget_simple_extent_dims(sp, &n, nullptr);
+        if (n != filePitch * icy::SimParams::nPtsArrays || tmp.nPts < 0 || (size_t)tmp.nPts > filePitch)
+            throw std::runtime_error("ReadSnapshot: Points size mismatch");
+    }
+    std::vector<double> pts(filePitch * icy::SimParams::nPtsArrays);
+    check(H5Dread(dpts, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, pts.data()), "reading Points");
+
+    const double viewSize = model->prms.ParticleViewSize;
+    model->prms = tmp;
+    model->prms.ParticleViewSize = viewSize;
+    model->gpu.cuda_allocate_arrays(model->prms.GridTotal, model->prms.nPts);
+    const size_t pitch = model->prms.nPtsPitch;
+    double *dst = model->gpu.tmp_transfer_buffer;
+    const int n = model->prms.nPts;
+    // row 0 holds bytes (crushed at i, grain at pitch + 2i, viewer byte at
+    // 3 pitch + i): re-pitch it byte-wise, the other rows as doubles
+    std::fill(dst, dst + pitch * icy::SimParams::nPtsArrays, 0.0);
+    const unsigned char *s8 = reinterpret_cast<const unsigned char *>(pts.data());
+    unsigned char *d8 = reinterpret_cast<unsigned char *>(dst);
+    for (int i = 0; i < n; ++i) {
+        d8[i] = s8[i];
+        d8[pitch + 2 * (size_t)i] = s8[filePitch + 2 * (size_t)i];
+        d8[pitch + 2 * (size_t)i + 1] = s8[filePitch + 2 * (size_t)i + 1];
+        d8[3 * pitch + i] = s8[3 * filePitch + i];
+    }
+    for (size_t r = 1; r < icy::SimParams::nPtsArrays; ++r)
+        std::copy(pts.begin() + r * filePitch, pts.begin() + r * filePitch + n, dst + r * pitch);
+    model->gpu.transfer_ponts_to_device();
+    model->Prepare();
+}
+
 extern "C" {
+
+int icempm_model_read_snapshot(icempm_model *m, const char *path) {
+    if (!m || !path) return ICEMPM_E_ARG;
+    return icempm_guarded(m, [&] { m->snapshot.ReadSnapshot(std::string(path)); });
+}
 
 int icempm_model_load_raw_points_file(icempm_model *m, const char *path) {
     if (!m || !path) return ICEMPM_E_ARG;

@@ -25,6 +25,7 @@ public:
     // libicempm_h5.so
     void LoadRawPoints(std::string fileName);
     void SaveSnapshot(std::string directory);
+    void ReadSnapshot(std::string fileName);   // resume from a snapshot
     void SavePQ(std::string directory);
     // the attributes SaveSnapshot and SavePQ attach to a dataset (an HDF5
     // hid_t; the reference takes an H5::DataSet)

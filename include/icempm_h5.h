@@ -26,6 +26,11 @@ int icempm_model_load_raw_points_file(icempm_model *m, const char *path);
  * SaveParametersAsAttributes, snapshotmanager.cpp:107-145), Points
  * f64[16 nPtsPitch] */
 int icempm_model_save_snapshot(icempm_model *m, const char *dir);
+/* Resume from a snapshot written by icempm_model_save_snapshot (the
+ * reference's ReadSnapshot, snapshotmanager.cpp:147-184, is commented out):
+ * parameters, arrays and points restored and uploaded; stepping continues
+ * from the snapshot's SimulationStep and SimulationTime */
+int icempm_model_read_snapshot(icempm_model *m, const char *path);
 /* SnapshotManager::SavePQ (snapshotmanager.cpp:14-56): <dir>/pq/pqNNNNN.h5
  * with PQ_intact f64[n][2] (with the attributes) and PQ_crushed f64[n][2] */
 int icempm_model_save_pq(icempm_model *m, const char *dir);

@@ -77,3 +77,31 @@ def test_snapshot_and_pq_files(gpu_available, tmp_path):
         assert ("attribute", "nPts", "H5T_STD_I32LE") in structure(pq)
     finally:
         m.close()
+
+
+def test_resume_from_snapshot_continues_bit_exact(gpu_available, tmp_path):
+    """ORDERED: two cycles straight equal one cycle, a snapshot, a new model
+    resumed from it and one more cycle."""
+    print("snapshot: resume", flush=True)
+    case = INDENT_TINY
+    a = Model(MODE_ORDERED)
+    b = Model(MODE_ORDERED)
+    try:
+        a.load_case(case)
+        a.step()
+        a.wait_cycle()
+        path = a.save_snapshot(str(tmp_path))
+        a.step()
+        a.wait_cycle()
+        configure_params(b.prms, case)
+        b.read_snapshot(path)
+        assert b.prms.SimulationStep == a.prms.UpdateEveryNthStep
+        b.step()
+        b.wait_cycle()
+        assert b.prms.SimulationStep == a.prms.SimulationStep
+        assert b.prms.SimulationTime == a.prms.SimulationTime
+        assert np.array_equal(b.host_points().view(np.uint64), a.host_points().view(np.uint64))
+        assert np.array_equal(b.host_force(), a.host_force())
+    finally:
+        a.close()
+        b.close()
