@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from icempm_amd import (INDENT_60K, INDENT_TINY, MODE_FAST, MODE_ORDERED, Model, SimParams,
+from icempm_amd import (INDENT_1M6, INDENT_60K, INDENT_TINY, MODE_FAST, MODE_ORDERED, Model, SimParams,
                         configure_params, lattice_points)
 from icempm_amd import _lib
 
@@ -226,3 +226,47 @@ def test_call_order_errors(gpu_available):
     assert b"constants" in H.icempm_last_error(ctx)
     assert H.icempm_update_constants(ctx, ctypes.byref(SimParams()), 100) == _lib.ICEMPM_E_ARG
     H.icempm_destroy(ctx)
+
+
+def test_full_size_case(gpu_available):
+    """The 1.6M-point configuration itself: ORDERED bit-exact for a few
+    substeps, FAST within the north-star tolerance, and the size-independent
+    conservation of a P2G (total mass N m; total momentum m sum v, since the
+    APIC affine term sums to zero over the stencil)."""
+    case = INDENT_1M6
+    say("full size ordered", case.name)
+    m, p, pts = load_pair(case, MODE_ORDERED)
+    m.run(6)
+    grid = oracle.new_grid(p)
+    bins = np.zeros(2 * p.n_indenter_subdivisions)
+    _, err = oracle.run(p, pts, grid, bins, 6)
+    assert err == 0 and m.error_flag() == 0
+    assert np.array_equal(m.points().view(np.uint64), pts.view(np.uint64))
+    m.close()
+
+    say("full size fast", case.name)
+    m, p, pts = load_pair(case, MODE_FAST)
+    nsub = 30
+    hist_dev = m.run(nsub)
+    grid = oracle.new_grid(p)
+    bins = np.zeros(2 * p.n_indenter_subdivisions)
+    hist_ref, err = oracle.run(p, pts, grid, bins, nsub)
+    assert err == 0 and m.error_flag() == 0
+    n = case.n_points
+    dev = m.points()
+    assert rel_l2(hist_dev, hist_ref) <= 1e-6
+    assert np.abs(dev[4:6, :n] - pts[4:6, :n]).max() <= 1e-12
+    assert np.abs(dev[8:12, :n] - pts[8:12, :n]).max() <= 1e-9
+    assert np.array_equal(dev[0].view(np.uint8)[:n], pts[0].view(np.uint8)[:n])
+
+    say("full size p2g conservation", case.name)
+    H = _lib.hip()
+    _lib.check_ctx(H.icempm_reset_grid(m.ctx), m.ctx)
+    _lib.check_ctx(H.icempm_p2g(m.ctx), m.ctx)
+    g = m.grid()
+    mass = m.prms.ParticleMass
+    assert g[0].sum() == pytest.approx(n * mass, rel=1e-12)
+    mom = mass * dev[6:8, :n].sum(axis=1)
+    scale = mass * np.abs(dev[6:8, :n]).sum() + 1e-300
+    assert np.abs(g[1:3].sum(axis=1) - mom).max() <= 1e-9 * scale
+    m.close()
