@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
     double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
     double B00 = 0, B01 = 0, B10 = 0, B11 = 0;
     bool defer = false;   // learned map: projection and P2G done by the map kernel
-    Trial rt{};           // FP64 storage: the trial state, and whether F is still its F
+    Inv rt{};             // FP64 storage: the trial invariants, and whether F is still their F
     bool fresh = false;
 
     if constexpr (G2P) {
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         const bool ok = valid && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
                         stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
         if (valid && !ok) flag_error(k);
-        Trial r{};
+        Inv iv{};
         unsigned char crushed0 = crushed;
         double Jp0 = Jp;
         if (ok) {
@@ -271,15 +271,15 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             const double n00 = M00 * F00 + M01 * F10, n01 = M00 * F01 + M01 * F11;
             const double n10 = M10 * F00 + M11 * F10, n11 = M10 * F01 + M11 * F11;
             F00 = n00; F01 = n01; F10 = n10; F11 = n11;
-            r = compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa);
+            iv = invariants_fast(F00, F01, F10, F11, cp.mu, cp.kappa);
             crushed0 = crushed;
             Jp0 = Jp;
             if (crushed == 0) {
                 bool f;
-                if (r.p < 0) {
-                    f = (r.p < -3e6) || (r.q > cp.slope * (r.p - -3e6));
+                if (iv.p < 0) {
+                    f = (iv.p < -3e6) || (iv.q > cp.slope * (iv.p - -3e6));
                 } else {
-                    f = cp.onePlus2Beta * r.q * r.q + cp.msq * (r.p + cp.betaPmax) * (r.p - cp.pmax) > 0;
+                    f = cp.onePlus2Beta * iv.q * iv.q + cp.msq * (iv.p + cp.betaPmax) * (iv.p - cp.pmax) > 0;
                 }
                 crushed = f ? 1 : 0;
             }
@@ -287,10 +287,15 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             B00 = b00; B01 = b01; B10 = b10; B11 = b11;
         }
         // ---- return map (convergent: the learned map runs wavefront-wide) ----
+        // The trial SVD only for crushed points (few: the wavefront mostly
+        // skips it); p and q for the NACC test came from the invariants.
         const bool crushedOk = ok && crushed == 1;
         if constexpr (RMAP == kMapExact) {
-            if (crushedOk) dp_return(k, r, F00, F01, F10, F11, Jp);
+            if (crushedOk && dp_projects(k, iv.p, iv.q, Jp))
+                dp_return(k, compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa), F00, F01, F10, F11, Jp);
         } else if constexpr (RMAP == kMapHarvest) {
+            Trial r{};
+            if (crushedOk) r = compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa);
             double f[5] = {0, 0, 0, 0, 0}, t[3] = {0, 0, 0};
             bool usable = false;
             if (crushedOk) {
@@ -324,6 +329,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             // kernel (network proposal, residual check, exact fallback) unless
             // their state is inverted, which takes the exact map here
             double qlim = 0;
+            Trial r{};
+            if (crushedOk) r = compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa);
             const int br = crushedOk ? map_branch(k, r, Jp, qlim) : kBrInside;
             const bool proj = crushedOk && br != kBrInside;
             defer = proj && r.svd.s1 > 0;
@@ -353,7 +360,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             }
         }
         if constexpr (!F32) {
-            rt = r;
+            rt = iv;
             fresh = ok && crushed == 0;
         }
         if (ok) {
@@ -422,7 +429,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         if (ok) {
             double t00, t01, t11;
             if (fresh)
-                kirchhoff_from_trial(rt, t00, t01, t11);
+                kirchhoff_from_inv(rt, t00, t01, t11);
             else
                 kirchhoff(F00, F01, F10, F11, cp.mu, cp.kappa, t00, t01, t11);
             const double m = cp.mass, h = cp.h;

@@ -228,6 +228,47 @@ static __device__ __forceinline__ void kirchhoff_from_trial(const Trial &r, doub
     t11 = ss * d0 + cc * d1;
 }
 
+// FAST-mode trial invariants without the SVD.  With J = det F and
+// b = F F^T, the squared singular values are b's eigenvalues, so
+// ss0 - ss1 = sqrt((b00 - b11)^2 + 4 b01^2) and
+//   p = -kappa/2 (J^2 - 1),  q = |mu / J| (ss0 - ss1)
+// (the same p and q as ComputePQ, gpu_implementation5.cu:36-44).  The NACC
+// test and the Kirchhoff stress need nothing else; only a point the
+// return map projects needs the SVD.
+struct Inv {
+    double c, d, b01, p, q;   // c = mu / J, d = b00 - b11
+};
+
+static __device__ __forceinline__ Inv invariants_fast(double F00, double F01, double F10, double F11,
+                                                      double mu, double kappa) {
+    Inv v;
+    const double J = F00 * F11 - F10 * F01;
+    const double b00 = F00 * F00 + F01 * F01, b11 = F10 * F10 + F11 * F11;
+    v.b01 = F00 * F10 + F01 * F11;
+    v.d = b00 - b11;
+    v.c = mu / J;
+    v.p = -(kappa / 2.) * (J * J - 1.);
+    v.q = fabs(v.c) * sqrt(v.d * v.d + 4. * (v.b01 * v.b01));
+    return v;
+}
+
+// tau = mu/J dev(b) + kappa (J^2 - 1) I from the invariants of the same F
+static __device__ __forceinline__ void kirchhoff_from_inv(const Inv &v, double &t00, double &t01,
+                                                          double &t11) {
+    const double a = 0.5 * (v.c * v.d);
+    t00 = a - 2. * v.p;
+    t01 = v.c * v.b01;
+    t11 = -a - 2. * v.p;
+}
+
+// Whether the Drucker-Prager map (dp_return) changes a crushed trial state
+// with these p and q: the tension branch, or q at or above the cone.
+static __device__ __forceinline__ bool dp_projects(const KParams &k, double p, double q, double Jp_inv) {
+    if (p < -k.dpThr || Jp_inv < 1) return true;
+    const double q_n_1 = (p > k.ics) ? 0.0 : fmin((p + k.dpThr) * k.tanPhi, k.iss);
+    return q >= q_n_1;
+}
+
 // NACC test with the constants of grain 0 (GrainVariability == 0)
 static __device__ __forceinline__ bool fails_g0(const KParams &k, double p, double q) {
     if (p < 0) return (p < -3e6) || (q > k.g0[4] * (p - -3e6));
