@@ -52,14 +52,49 @@ struct Rows {
     size_t pitch;
 };
 
-static __device__ __forceinline__ int wmin(int v) {
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 static __device__ __forceinline__ void wsync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+}
+
+// Minimum / maximum over the wavefront of the two 16-bit halves of a word:
+// DPP permutations within each row of 16 lanes (pairs, quads, half-row
+// mirror, row mirror), then the four row results through readlane.  Every
+// lane of the wavefront must be active.
+typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));
+template <bool MAX>
+static __device__ __forceinline__ unsigned pk_sel(unsigned a, unsigned b) {
+    const U16x2 x = __builtin_bit_cast(U16x2, a), y = __builtin_bit_cast(U16x2, b);
+    return __builtin_bit_cast(unsigned, MAX ? __builtin_elementwise_max(x, y) : __builtin_elementwise_min(x, y));
+}
+template <bool MAX>
+static __device__ __forceinline__ unsigned wreduce_u16x2(unsigned v) {
+    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    const unsigned r0 = (unsigned)__builtin_amdgcn_readlane((int)v, 0), r1 = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+    const unsigned r2 = (unsigned)__builtin_amdgcn_readlane((int)v, 32), r3 = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+    return pk_sel<MAX>(pk_sel<MAX>(r0, r1), pk_sel<MAX>(r2, r3));
+}
+
+// Band order of a case's base cells: the rows in pairs (bands, a last odd
+// row alone), a band column by column with the two cells of a column
+// together.  Consecutive cells then cover two rows: a wavefront's 64 points
+// (16 cells at 4 per cell) lie in 2 x 8 cells, whose stencils touch 4 x 10
+// grid nodes, where 1 x 16 cells of a row-major order touch 3 x 18.  The
+// keys of a case are a bijection onto [0, gx * gy).
+static __host__ __device__ inline int band_key(int x, int y, int gx, int gy) {
+    const int band = y >> 1;
+    const int rows = (gy - 2 * band) < 2 ? 1 : 2;
+    return band * 2 * gx + x * rows + (y & 1);
+}
+static __host__ __device__ inline void band_cell(int key, int gx, int gy, int &x, int &y) {
+    const int band = key / (2 * gx);
+    const int w = key - band * 2 * gx;
+    const int rows = (gy - 2 * band) < 2 ? 1 : 2;
+    x = w / rows;
+    y = 2 * band + w % rows;
 }
 
 static __device__ __forceinline__ KParams kp_of(const CaseParams &c, int *err) {
@@ -415,15 +450,31 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         const bool ok = have && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
                         stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
         if (valid && !ok && !defer) flag_error(k);   // deferred points scatter in the map kernel
-        const int ox = wmin(ok ? bx : INT_MAX);
-        const int oy = wmin(ok ? by : INT_MAX);
+        // the box of base cells of the lanes whose stencils can reach the
+        // window (a lane outside the grid touches no node): the window's
+        // origin and the w x hgt nodes the stencils touch, which alone are
+        // cleared and flushed; lanes map onto the box 16, 32 or 64 columns
+        // at a time
+        const bool inb = ok && bx < gx && by < gy;
+        const bool any = __ballot(inb) != 0;
+        const unsigned cell = (unsigned)bx | ((unsigned)by << 16);
+        const unsigned lo = wreduce_u16x2<false>(inb ? cell : 0xFFFFFFFFu);
+        const unsigned hi = wreduce_u16x2<true>(inb ? cell : 0u);
+        const int ox = any ? (int)(lo & 0xFFFFu) : INT_MAX;
+        const int oy = any ? (int)(lo >> 16) : INT_MAX;
+        const int bw = any ? min((int)(hi & 0xFFFFu) + 3 - ox, kWX) : 0;
+        const int bh = any ? min((int)(hi >> 16) + 3 - oy, kWY) : 0;
+        const int bsh = bw <= 16 ? 4 : (bw <= 32 ? 5 : 6);
+        const int bcol = lane & ((1 << bsh) - 1), brow0 = lane >> bsh, bstep = 64 >> bsh;
         double *W0 = smem[wv];
         double *W1 = W0 + kWY * kWS, *W2 = W1 + kWY * kWS;
-        for (int c2 = lane; c2 < kWX * kWY; c2 += 64) {
-            const int e = (c2 / kWX) * kWS + c2 % kWX;
-            W0[e] = 0;
-            W1[e] = 0;
-            W2[e] = 0;
+        if (bcol < bw) {
+            for (int ly = brow0; ly < bh; ly += bstep) {
+                const int e = ly * kWS + bcol;
+                W0[e] = 0;
+                W1[e] = 0;
+                W2[e] = 0;
+            }
         }
         wsync();
         if (ok) {
@@ -438,7 +489,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             const double A10 = m * B10 - sc * t01, A11 = m * B11 - sc * t11;
             const double mv0 = vx * m, mv1 = vy * m;
             const int lx0 = bx - ox, ly0 = by - oy;
-            if (bx + 2 < gx && by + 2 < gy && lx0 + 2 < kWX && ly0 + 2 < kWY) {
+            // unsigned: a stencil left of or below the origin never lands in the window
+            if (bx + 2 < gx && by + 2 < gy && (unsigned)lx0 + 2 < (unsigned)kWX &&
+                (unsigned)ly0 + 2 < (unsigned)kWY) {
                 const int e0 = ly0 * kWS + lx0;
                 double xw0 = wx0, xw1 = wx1, xw2 = wx2;
                 double xa0 = mv0 + A00 * ((0 - fx0) * h), xa1 = mv0 + A00 * ((1 - fx0) * h),
@@ -486,7 +539,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
                             flag_error(k);
                             continue;
                         }
-                        if (X - ox < kWX && Y - oy < kWY) {
+                        if ((unsigned)(X - ox) < (unsigned)kWX && (unsigned)(Y - oy) < (unsigned)kWY) {
                             const int e = (Y - oy) * kWS + (X - ox);
                             atomicAdd(&W0[e], incM);
                             atomicAdd(&W1[e], incV0);
@@ -501,9 +554,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
             }
         }
         wsync();
-        if (ox != INT_MAX) {
-            for (int c2 = lane; c2 < kWX * kWY; c2 += 64) {
-                const int lx = c2 % kWX, ly = c2 / kWX;
+        if (bcol < bw) {
+            for (int ly = brow0; ly < bh; ly += bstep) {
+                const int lx = bcol;
                 const int e = ly * kWS + lx;
                 const double a = W0[e];
                 if (a != 0) {
@@ -761,7 +814,7 @@ __global__ void k_batch_keys(const double *__restrict__ pos, size_t pitch, int n
     if (x == x && y == y) {
         const double bx = fmin(fmax(x * cp.h_inv - 0.5, 0.0), (double)(cp.gx - 1));
         const double by = fmin(fmax(y * cp.h_inv - 0.5, 0.0), (double)(cp.gy - 1));
-        key = (int)bx + (int)by * cp.gx;
+        key = band_key((int)bx, (int)by, cp.gx, cp.gy);
     }
     keys[i] = cp.gridOffset + key;
     vals[i] = i;
@@ -959,12 +1012,11 @@ hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const double *gV, do
     return hipGetLastError();
 }
 
-// Chunk table from the case-local cell keys of the points in slot order:
-// chunks of <= chunkPoints consecutive points of one case whose base cells
-// span at most two cell rows and 126 columns, so that a wavefront's window
-// (4 rows) holds all its stencils.  Two rows, not one: the ingest sorts by
-// the cell of the raw coordinates, and after the block offset the points of
-// one raw row can interleave two rows (the first raw row always does).
+// Chunk table from the case-local band keys of the base cells of the points
+// in slot order: chunks of <= chunkPoints consecutive points of one case
+// whose base cells span at most two cell rows and 126 columns, so that a
+// wavefront's window (4 node rows) holds all its stencils.  Points in band
+// order give chunks of one band each, until they move.
 void chunk_table(const icempm_batch *b, const std::vector<int> &keys, std::vector<int> &cstart,
                  std::vector<int> &ccase) {
     cstart.clear();
@@ -972,15 +1024,18 @@ void chunk_table(const icempm_batch *b, const std::vector<int> &keys, std::vecto
     const int cap = b->chunkPoints;
     for (size_t ci = 0; ci < b->hc.size(); ++ci) {
         const CaseParams &cp = b->hc[ci].cp;
-        const int n = cp.nPts, gx = cp.gx, o = cp.ptOffset;
+        const int n = cp.nPts, gx = cp.gx, gy = cp.gy, o = cp.ptOffset;
         if (n <= 0) continue;
         int cs = 0;
         cstart.push_back(o);
         ccase.push_back((int)ci);
-        int r0 = keys[o] / gx, r1 = r0, c0 = keys[o] % gx, c1 = c0;
+        int c0, r0;
+        band_cell(keys[o], gx, gy, c0, r0);
+        int r1 = r0, c1 = c0;
         for (int i = 1; i < n; ++i) {
             const int ki = keys[(size_t)o + i];
-            const int r = ki / gx, cc = ki % gx;
+            int cc, r;
+            band_cell(ki, gx, gy, cc, r);
             const int nr0 = std::min(r0, r), nr1 = std::max(r1, r), nc0 = std::min(c0, cc), nc1 = std::max(c1, cc);
             const bool brk = (i - cs >= cap) || nr1 - nr0 > 1 || nc1 - nc0 > 125 || ki < 0 || keys[(size_t)o + cs] < 0;
             if (brk) {
@@ -1194,6 +1249,8 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
     std::memcpy(&P, simparams, sizeof(P));
     if (P.nPts <= 0 || (size_t)P.nPts > pitch || P.GridX <= 0 || P.GridY <= 0)
         return -bfail(b, ICEMPM_E_ARG, "add_case: bad sizes");
+    if (P.GridX > 32767 || P.GridY > 32767)   // cell indices travel as 16-bit halves in the particle pass
+        return -bfail(b, ICEMPM_E_ARG, "add_case: grid wider or taller than 32767 cells");
     if (P.GrainVariability != 0)
         return -bfail(b, ICEMPM_E_ARG, "add_case: a batched case needs a uniform material (GrainVariability = 0)");
     if (!b->hc.empty() && P.InitialTimeStep != b->hc[0].cp.dt)
@@ -1229,7 +1286,15 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
         for (int r = 0; r < 10; ++r) h.rest[(size_t)r * n + i] = soa[pitch * (6 + r) + i];
         h.rest[10 * (size_t)n + i] = soa[pitch * 3 + i];
         h.crushed[i] = u8[i];
-        h.keys[i] = (int)(h.pos[i] * c.h_inv - 0.5) + ((int)(h.pos[n + i] * c.h_inv - 0.5)) * c.gx;
+        // the base cell's band key, clamped into the grid as on the device
+        const double x = h.pos[i], y = h.pos[n + i];
+        int key = c.gridTotal - 1;
+        if (x == x && y == y) {
+            const double bx = std::fmin(std::fmax(x * c.h_inv - 0.5, 0.0), (double)(c.gx - 1));
+            const double by = std::fmin(std::fmax(y * c.h_inv - 0.5, 0.0), (double)(c.gy - 1));
+            key = band_key((int)bx, (int)by, c.gx, c.gy);
+        }
+        h.keys[i] = key;
     }
     b->hc.push_back(std::move(h));
     return (int)b->hc.size() - 1;
@@ -1242,11 +1307,18 @@ int icempm_batch_finalize(icempm_batch *b) {
     std::vector<int2> nblk;
     std::vector<int> keys;
     long pofs = 0, gofs = 0;
+    // every case's points in band order (stable): slot o + k holds point
+    // order[k] of the case, and orig maps it back
+    std::vector<std::vector<int>> order(b->hc.size());
     for (size_t ci = 0; ci < b->hc.size(); ++ci) {
         auto &h = b->hc[ci];
         h.cp.ptOffset = (int)pofs;
         h.cp.gridOffset = (int)gofs;
-        keys.insert(keys.end(), h.keys.begin(), h.keys.end());
+        auto &ord = order[ci];
+        ord.resize((size_t)h.cp.nPts);
+        for (int i = 0; i < h.cp.nPts; ++i) ord[(size_t)i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&h](int a, int c) { return h.keys[(size_t)a] < h.keys[(size_t)c]; });
+        for (int k = 0; k < h.cp.nPts; ++k) keys.push_back(h.keys[(size_t)ord[(size_t)k]]);
         for (int nd = 0; nd < h.cp.gridTotal; nd += 256) nblk.push_back(make_int2((int)ci, nd));
         pofs += h.cp.nPts;
         gofs += h.cp.gridTotal;
@@ -1275,21 +1347,23 @@ int icempm_batch_finalize(icempm_batch *b) {
     std::vector<unsigned char> hrest(11 * b->pitch * vs, 0), hcr(b->pitch, 0);
     std::vector<int> horig(b->pitch, 0);
     std::vector<CaseParams> hcp;
-    for (auto &h : b->hc) {
+    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
+        const auto &h = b->hc[ci];
         const int n = h.cp.nPts, o = h.cp.ptOffset;
-        for (int i = 0; i < n; ++i) {
-            hpos[o + i] = h.pos[i];
-            hpos[b->pitch + o + i] = h.pos[n + i];
-            hcr[o + i] = h.crushed[i];
-            horig[o + i] = o + i;
+        for (int k = 0; k < n; ++k) {
+            const int i = order[ci][(size_t)k];
+            hpos[o + k] = h.pos[i];
+            hpos[b->pitch + o + k] = h.pos[n + i];
+            hcr[o + k] = h.crushed[i];
+            horig[o + k] = o + i;
             for (int r = 0; r < 11; ++r) {
                 double v = h.rest[(size_t)r * n + i];
                 if (b->precision == ICEMPM_BATCH_F32) {
                     if (r == 2 || r == 5) v -= 1.0;    // Fe00, Fe11 stored as Fe - I
                     float f = (float)v;
-                    std::memcpy(&hrest[((size_t)r * b->pitch + o + i) * vs], &f, vs);
+                    std::memcpy(&hrest[((size_t)r * b->pitch + o + k) * vs], &f, vs);
                 } else {
-                    std::memcpy(&hrest[((size_t)r * b->pitch + o + i) * vs], &v, vs);
+                    std::memcpy(&hrest[((size_t)r * b->pitch + o + k) * vs], &v, vs);
                 }
             }
         }
