@@ -58,9 +58,15 @@ def _run(cmd: list[str]) -> None:
             sys.stderr.write("\n".join(lines) + "\n")
 
 
+# per-source flags: the batched particle pass takes its leading scalar
+# arguments preloaded in SGPRs (gfx950 kernarg preload; the code object keeps
+# a prologue that loads them where the firmware does not)
+HIP_EXTRA = {"batch.hip": ["-mllvm", "-amdgpu-kernarg-preload-count=16"]}
+
+
 def _compile_hip(src: str) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    _run([HIPCC, *HIP_FLAGS, *HIP_EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
     return obj
 
 

@@ -198,21 +198,28 @@ static __device__ __forceinline__ double yield_residual(const KParams &k, const 
 constexpr int kWaveWinBytes = 3 * kWY * kWS * 8;
 
 // Fused particle pass over all cases (G2P of substep k and/or P2G of k+1).
+// The chunk table, the case table and the point rows lead the arguments,
+// which the kernel receives preloaded in SGPRs: a workgroup's first load is
+// its chunk {first point, end, case}, the point loads follow it at once,
+// and the case's parameters arrive meanwhile.
 template <bool F32, bool G2P, bool P2G, int RMAP>
-__global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const CaseParams *__restrict__ cases,
-                                                         const int *__restrict__ chunkStart,
-                                                         const int *__restrict__ chunkCase,
-                                                         int nChunks, const double *__restrict__ gridV,
+__global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict__ chunks, int nChunks,
+                                                         const CaseParams *__restrict__ cases, double *pos,
+                                                         typename Rows<F32>::V *rest, unsigned char *crushedRow,
+                                                         size_t pitch, const double *__restrict__ gridV,
                                                          double *__restrict__ gridOut, size_t gpitch,
                                                          int *__restrict__ err, LearnArgs la, MapQueue mq) {
+    const Rows<F32> S{pos, rest, crushedRow, pitch};
     __shared__ __attribute__((aligned(16))) double smem[4][kWaveWinBytes / 8];
     const int per = (nChunks + 7) >> 3;
     const int c = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (c >= nChunks) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int beg = chunkStart[c], end = chunkStart[c + 1];
+    const int4 ch = chunks[c];
+    const int beg = ch.x, end = ch.y;
+    asm volatile("" ::"s"(ch.z));   // all of the chunk in one load, before the branch
     if (beg + wv * 64 >= end) return;
-    const CaseParams cp = cases[chunkCase[c]];
+    const CaseParams cp = cases[ch.z];
     const KParams k = kp_of(cp, err);
     const int pt = beg + (int)threadIdx.x;
     const bool valid = pt < end;
@@ -380,7 +387,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
                     if (slot < (unsigned int)mq.cap) {
                         MapEntry &e = mq.q[slot];
                         e.pt = pt;
-                        e.cs = chunkCase[c];
+                        e.cs = ch.z;
                         e.F[0] = F00; e.F[1] = F01; e.F[2] = F10; e.F[3] = F11;
                         e.Jp = Jp;
                     } else {
@@ -401,22 +408,28 @@ __global__ __launch_bounds__(256) void k_batch_particles(Rows<F32> S, const Case
         if (ok) {
             S.pos[pt] = px;
             S.pos[pp + pt] = py;
-            R[pt] = (V)vx;
-            R[pp + pt] = (V)vy;
             if (!defer) {
                 R[pp * 2 + pt] = (V)(F00 - fe0);
                 R[pp * 3 + pt] = (V)F01;
                 R[pp * 4 + pt] = (V)F10;
                 R[pp * 5 + pt] = (V)(F11 - fe0);
             }
-            R[pp * 6 + pt] = (V)B00;
-            R[pp * 7 + pt] = (V)B01;
-            R[pp * 8 + pt] = (V)B10;
-            R[pp * 9 + pt] = (V)B11;
+            // v and Bp: the fused pass uses them at once and the next G2P
+            // recomputes them before any use, so it stores them only for
+            // points the map kernel scatters; the G2P-only pass that ends a
+            // run stores them for every point
+            if (!P2G || defer) {
+                R[pt] = (V)vx;
+                R[pp + pt] = (V)vy;
+                R[pp * 6 + pt] = (V)B00;
+                R[pp * 7 + pt] = (V)B01;
+                R[pp * 8 + pt] = (V)B10;
+                R[pp * 9 + pt] = (V)B11;
+            }
             if (Jp != Jp0 && !defer) R[pp * 10 + pt] = (V)Jp;
             if (crushed != crushed0) S.crushed[pt] = crushed;
             if constexpr (F32) {
-                // continue the P2G from what was stored, as a separate pass would
+                // continue the P2G from the stored precision, as a separate pass would
                 vx = (double)(V)vx; vy = (double)(V)vy;
                 F00 = fe0 + (double)(V)(F00 - fe0); F01 = (double)(V)F01;
                 F10 = (double)(V)F10; F11 = fe0 + (double)(V)(F11 - fe0);
@@ -866,7 +879,7 @@ struct icempm_batch {
     size_t gpitch = 0;
     long nNodes = 0;
     CaseParams *cases = nullptr;
-    int *chunkStart = nullptr, *chunkCase = nullptr;
+    int4 *chunks = nullptr;   // {first point, end, case, 0} per chunk
     int nChunks = 0;
     size_t chunkCapDev = 0;
     int2 *nodeBlock = nullptr;
@@ -957,8 +970,9 @@ template <bool F32, bool G2P, bool P2G, int RMAP>
 void launch_one(icempm_batch *b, unsigned nb, const double *gV, double *gO, const LearnArgs &la,
                 const MapQueue &mq) {
     Rows<F32> S{b->pos, reinterpret_cast<typename Rows<F32>::V *>(b->rest), b->crushed, b->pitch};
-    hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream, S,
-                       b->cases, b->chunkStart, b->chunkCase, b->nChunks, gV, gO, b->gpitch, b->err, la, mq);
+    hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream,
+                       b->chunks, b->nChunks, b->cases, S.pos, S.rest, S.crushed, S.pitch, gV, gO, b->gpitch,
+                       b->err, la, mq);
     if constexpr (G2P && RMAP == kMapLearned) {
         // the deferred points: map, store, and their P2G into the pass's output grid
         hipLaunchKernelGGL((k_batch_map<F32>), dim3(kMapBlocks), dim3(256), 0, b->stream, S, b->cases, mq.q,
@@ -1058,21 +1072,19 @@ void drop_graph(icempm_batch *b) {
     b->graphSteps = 0;
 }
 
-// the host chunk table to the device (grows the device arrays when needed)
+// the host chunk table to the device (grows the device array when needed)
 int upload_chunks(icempm_batch *b) {
-    const size_t need = b->hcstart.size();
-    if (need > b->chunkCapDev) {
-        dfree(b->chunkStart);
-        dfree(b->chunkCase);
+    const size_t n = b->hccase.size();
+    std::vector<int4> t(std::max<size_t>(n, 1));
+    for (size_t c = 0; c < n; ++c) t[c] = make_int4(b->hcstart[c], b->hcstart[c + 1], b->hccase[c], 0);
+    if (t.size() > b->chunkCapDev) {
+        dfree(b->chunks);
         b->chunkCapDev = 0;
-        BCHK(b, hipMalloc(&b->chunkStart, need * sizeof(int)), "chunk table");
-        BCHK(b, hipMalloc(&b->chunkCase, need * sizeof(int)), "chunk table");
-        b->chunkCapDev = need;
+        BCHK(b, hipMalloc(&b->chunks, t.size() * sizeof(int4)), "chunk table");
+        b->chunkCapDev = t.size();
     }
-    BCHK(b, hipMemcpy(b->chunkStart, b->hcstart.data(), need * sizeof(int), hipMemcpyHostToDevice), "chunk table");
-    BCHK(b, hipMemcpy(b->chunkCase, b->hccase.data(), b->hccase.size() * sizeof(int), hipMemcpyHostToDevice),
-         "chunk table");
-    b->nChunks = (int)b->hccase.size();
+    BCHK(b, hipMemcpy(b->chunks, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice), "chunk table");
+    b->nChunks = (int)n;
     drop_graph(b);
     return ICEMPM_OK;
 }
@@ -1225,7 +1237,7 @@ void icempm_batch_destroy(icempm_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     dfree(b->pos); dfree(b->rest); dfree(b->crushed); dfree(b->gridA); dfree(b->gridB);
-    dfree(b->cases); dfree(b->chunkStart); dfree(b->chunkCase); dfree(b->nodeBlock);
+    dfree(b->cases); dfree(b->chunks); dfree(b->nodeBlock);
     dfree(b->err); dfree(b->tTab); dfree(b->hist);
     dfree(b->mlpW); dfree(b->mlpP); dfree(b->hx); dfree(b->ht); dfree(b->hcount); dfree(b->stats);
     dfree(b->mapQ); dfree(b->mapCount);
