@@ -33,22 +33,28 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of the fused G2P+P2G pass (FP64):
-#   per point: read pos 16 + Fe 32 + Jp 8 + crushed 1 + grain 2 = 59 B,
-#              write pos 16 + vel 16 + Fe 32 + Bp 32 + Jp 8 + crushed 1 = 105 B
-#   per grid node: read velocity 16 B (G2P) + add mass/momentum 24 B (P2G)
-FUSED_BYTES_PER_POINT = 164
+# Algorithmic bytes per particle-update (the roofline numerator): the point
+# state read and written once per substep, as the reference's algorithm
+# keeps it in memory between substeps -- FP64: read pos 16 + vel 16 + Fe 32
+# + Bp 32 + Jp 8 + crushed 1 + grain 2 = 107 B, write the same less grain =
+# 105 B, 212 B in all; FP32 state: 55 + 53 = 108 B.  A fused pass performs
+# one particle-update per point.  Per grid node the fused pass's share of
+# the substep is the P2G result written (24 B) and the G2P velocity read
+# (16 B); the grid update kernel has the other 40 B.
+FUSED_BYTES_PER_POINT = 212
 FUSED_BYTES_PER_NODE = 40
-# whole-substep algorithmic bytes (point state read + written once: 212 B;
-# grid: P2G out 24 + update in 24 / out 16 + G2P in 16 = 80 B per node)
 SUBSTEP_BYTES_PER_POINT = 212
 SUBSTEP_BYTES_PER_NODE = 80
-# the batched pass with FP32 storage (positions FP64):
-#   per point: read pos 16 + Fe-I 16 + Jp 4 + crushed 1 = 37 B,
-#              write pos 16 + vel 8 + Fe-I 16 + Bp 16 = 56 B (Jp, crushed
-#              only when they change)
-BATCH_F32_BYTES_PER_POINT = 93
-BATCH_F64_BYTES_PER_POINT = 16 + 32 + 8 + 1 + 16 + 16 + 32 + 32
+BATCH_F32_BYTES_PER_POINT = 108
+BATCH_F64_BYTES_PER_POINT = 212
+# The bytes the fused passes actually have to move are fewer: the next G2P
+# recomputes v and Bp before any use, so a fused pass neither reads nor
+# writes them, and Jp and the crushed flag are written only when they
+# change.  FP64 single case: read pos 16 + Fe 32 + Jp 8 + crushed 1 + grain
+# 2, write pos 16 + Fe 32 = 107 B; batched FP32 storage (positions FP64, no
+# grain row): read 16 + 16 + 4 + 1, write 16 + 16 = 69 B.
+FUSED_MOVED_BYTES_PER_POINT = 107
+BATCH_F32_MOVED_BYTES_PER_POINT = 69
 METRIC = "particle-updates/s on 1 GPU: batched+learned vs plain MPM at ≤1% force err"
 
 
@@ -163,6 +169,7 @@ def run_single(args) -> dict:
     ms_per_step = el / steps * 1e3
     k_avg_s = (ktot / max(klaunch, 1)) / 1e3
     k_bytes = n * FUSED_BYTES_PER_POINT + G * FUSED_BYTES_PER_NODE
+    k_moved = n * FUSED_MOVED_BYTES_PER_POINT + G * FUSED_BYTES_PER_NODE
     achieved = k_bytes / k_avg_s / 1e9 if klaunch else None
     sub_bytes = n * SUBSTEP_BYTES_PER_POINT + G * SUBSTEP_BYTES_PER_NODE
     log(f"bench: {ms_per_step * 1e3:.1f} us/substep, fused kernel {k_avg_s * 1e6:.1f} us "
@@ -210,6 +217,8 @@ def run_single(args) -> dict:
             "traffic": traffic_from_profiles(f"fused_g2p_p2g_{case.name}"),
             "atomics": atomics_from_profiles(f"fused_g2p_p2g_{case.name}", k_avg_s),
             "bytes_per_launch": k_bytes,
+            "moved_bytes_per_launch": k_moved,
+            "moved_gbs": (k_moved / k_avg_s / 1e9) if k_avg_s else None,
             "avg_launch_us": k_avg_s * 1e6,
             "timed_launches": klaunch,
             "copy_kernel_gbs": copy_gbs,
@@ -297,6 +306,7 @@ def run_batch_learned(args) -> dict:
     b.close()
     k_avg_s = (ktot / max(klaunch, 1)) / 1e3
     k_bytes = n_total * BATCH_F32_BYTES_PER_POINT + nodes * FUSED_BYTES_PER_NODE
+    k_moved = n_total * BATCH_F32_MOVED_BYTES_PER_POINT + nodes * FUSED_BYTES_PER_NODE
     achieved = k_bytes / k_avg_s / 1e9 if klaunch else None
     log(f"bench: fused pass {k_avg_s * 1e6:.1f} us ({klaunch} timed launches), "
         f"{achieved:.0f} GB/s algorithmic")
@@ -379,6 +389,8 @@ def run_batch_learned(args) -> dict:
             "traffic": traffic_from_profiles("batch_particles_learned_32cases"),
             "atomics": atomics_from_profiles("batch_particles_learned_32cases", k_avg_s),
             "bytes_per_launch": k_bytes,
+            "moved_bytes_per_launch": k_moved,
+            "moved_gbs": (k_moved / k_avg_s / 1e9) if k_avg_s else None,
             "avg_launch_us": k_avg_s * 1e6,
             "timed_launches": klaunch,
             "copy_kernel_gbs": copy_gbs,

@@ -137,6 +137,7 @@ struct MapEntry {
     int pt, cs;
     double F[4];
     double Jp;
+    double x[2], v[2], B[4];   // position, velocity and Bp for its P2G (v, Bp at storage precision)
 };
 
 // Queue of deferred points of one pass (learned map only).
@@ -390,6 +391,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                         e.cs = ch.z;
                         e.F[0] = F00; e.F[1] = F01; e.F[2] = F10; e.F[3] = F11;
                         e.Jp = Jp;
+                        e.x[0] = px; e.x[1] = py;
+                        e.v[0] = (double)(V)vx; e.v[1] = (double)(V)vy;
+                        e.B[0] = (double)(V)B00; e.B[1] = (double)(V)B01;
+                        e.B[2] = (double)(V)B10; e.B[3] = (double)(V)B11;
                     } else {
                         flag_error(k);
                     }
@@ -414,11 +419,11 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 R[pp * 4 + pt] = (V)F10;
                 R[pp * 5 + pt] = (V)(F11 - fe0);
             }
-            // v and Bp: the fused pass uses them at once and the next G2P
-            // recomputes them before any use, so it stores them only for
-            // points the map kernel scatters; the G2P-only pass that ends a
-            // run stores them for every point
-            if (!P2G || defer) {
+            // v and Bp: the fused pass uses them at once (a deferred point's
+            // go to the map kernel in its queue entry) and the next G2P
+            // recomputes them before any use; the G2P-only pass that ends a
+            // run stores them
+            if constexpr (!P2G) {
                 R[pt] = (V)vx;
                 R[pp + pt] = (V)vy;
                 R[pp * 6 + pt] = (V)B00;
@@ -586,21 +591,18 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
     }
 }
 
-// P2G of one point from its stored position, velocity and Bp and the given
-// Fe (rounded to the storage type first, as the pass continues from what it
-// stored), straight to the grid with global atomics.
+// P2G of a deferred point from the position, velocity and Bp its queue entry
+// carries and the given Fe (rounded to the storage type first, as the pass
+// continues from what it stores), straight to the grid with global atomics.
 template <bool F32>
-static __device__ void scatter_point(const Rows<F32> &S, const CaseParams &cp, const KParams &k, int pt,
+static __device__ void scatter_point(const MapEntry &e, const CaseParams &cp, const KParams &k,
                                      double F00, double F01, double F10, double F11,
                                      double *__restrict__ gridOut, size_t gpitch) {
     using V = typename Rows<F32>::V;
     constexpr double fe0 = F32 ? 1.0 : 0.0;
-    const size_t pp = S.pitch;
-    const V *__restrict__ R = S.rest;
-    const double px = S.pos[pt], py = S.pos[pp + pt];
-    const double vx = (double)R[pt], vy = (double)R[pp + pt];
-    const double B00 = (double)R[pp * 6 + pt], B01 = (double)R[pp * 7 + pt];
-    const double B10 = (double)R[pp * 8 + pt], B11 = (double)R[pp * 9 + pt];
+    const double px = e.x[0], py = e.x[1];
+    const double vx = e.v[0], vy = e.v[1];
+    const double B00 = e.B[0], B01 = e.B[1], B10 = e.B[2], B11 = e.B[3];
     if constexpr (F32) {
         F00 = fe0 + (double)(V)(F00 - fe0); F01 = (double)(V)F01;
         F10 = (double)(V)F10; F11 = fe0 + (double)(V)(F11 - fe0);
@@ -724,7 +726,7 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
             R[pp * 4 + pt] = (V)F10;
             R[pp * 5 + pt] = (V)(F11 - fe0);
             if (Jp != e.Jp) R[pp * 10 + pt] = (V)Jp;
-            if (gridOut) scatter_point<F32>(S, cp, k, pt, F00, F01, F10, F11, gridOut, gpitch);
+            if (gridOut) scatter_point<F32>(e, cp, k, F00, F01, F10, F11, gridOut, gpitch);
         }
     }
 }
@@ -940,6 +942,7 @@ struct icempm_batch {
     // device timing of the fused particle pass (stream launches only)
     int timingStride = 0;
     std::vector<hipEvent_t> tev;   // pairs
+    hipEvent_t evAfter = nullptr;  // while set: recorded after the next particle kernel
     size_t tevNext = 0;
     double timedMs = 0;
     int timedLaunches = 0;
@@ -973,6 +976,7 @@ void launch_one(icempm_batch *b, unsigned nb, const double *gV, double *gO, cons
     hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream,
                        b->chunks, b->nChunks, b->cases, S.pos, S.rest, S.crushed, S.pitch, gV, gO, b->gpitch,
                        b->err, la, mq);
+    if (b->evAfter) (void)hipEventRecord(b->evAfter, b->stream);
     if constexpr (G2P && RMAP == kMapLearned) {
         // the deferred points: map, store, and their P2G into the pass's output grid
         hipLaunchKernelGGL((k_batch_map<F32>), dim3(kMapBlocks), dim3(256), 0, b->stream, S, b->cases, mq.q,
@@ -1201,8 +1205,10 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
                 b->tevNext += 2;
                 if ((e = hipEventRecord(t0, b->stream)) != hipSuccess) return e;
             }
-            if ((e = launch_pass(b, true, true, A, B, i)) != hipSuccess) return e;
-            if (timeIt && (e = hipEventRecord(t1, b->stream)) != hipSuccess) return e;
+            b->evAfter = timeIt ? t1 : nullptr;   // recorded right after the particle kernel
+            e = launch_pass(b, true, true, A, B, i);
+            b->evAfter = nullptr;
+            if (e != hipSuccess) return e;
             std::swap(A, B);
         } else {
             if ((e = launch_pass(b, true, false, A, nullptr, i)) != hipSuccess) return e;

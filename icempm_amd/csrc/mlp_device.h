@@ -56,40 +56,53 @@ static __device__ __forceinline__ f4 tile_bits(FA a, FB b, int K, f4 acc) {
 }
 
 // Inference scratch of one wavefront: 16 rows, activations as BF16 bits.
-// Rows are padded by one dword so that the 16 rows of an operand fetch fall
-// in different LDS banks.
-constexpr int kInLd = kIn + 2, kHLd = kH + 2;
+// Rows are padded by two dwords: a lane's four operand values stay one
+// aligned 8-byte read, and the 16 rows of a fetch spread over the banks.
+constexpr int kInLd = kIn + 4, kHLd = kH + 4;
 struct InferScratch {
-    unsigned short x[16][kInLd];
-    unsigned short h1[16][kHLd];
-    unsigned short h2[16][kHLd];
-    float y[16][kOut + 1];
+    alignas(16) unsigned short x[16][kInLd];
+    alignas(16) unsigned short h1[16][kHLd];
+    alignas(16) unsigned short h2[16][kHLd];
+    alignas(16) float y[16][kOut + 1];
 };
 
-// Forward of the 16 rows in s.x.  Wbf: parameters as BF16 bits (weights),
-// Pf: the same parameters as FP32 (biases are added in FP32).
+// One 16x16 output tile of A(16 x K) * W^T, A row-major with stride LD and
+// W[n][k] row-major with stride K (both BF16 bits, 8-byte aligned rows):
+// each lane's operands are one 8-byte read apiece.
+template <int K, int LD>
+static __device__ __forceinline__ f4 tile_rows(const unsigned short *A, const unsigned short *W, f4 acc) {
+    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const s4 av = *reinterpret_cast<const s4 *>(A + r * LD + k0 + 4 * g);
+        const s4 bv = *reinterpret_cast<const s4 *>(W + r * K + k0 + 4 * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Forward of the 16 rows in s.x.  Wbf: parameters as BF16 bits (weights;
+// 8-byte aligned), Pf: the same parameters as FP32 (biases are added in FP32).
 static __device__ __forceinline__ void infer16(InferScratch &s, const unsigned short *__restrict__ Wbf,
                                                const float *__restrict__ Pf) {
+    static_assert(kInLd % 4 == 0 && kHLd % 4 == 0 && oW1 % 4 == 0 && oW2 % 4 == 0 && oW3 % 4 == 0,
+                  "8-byte operand reads");
     const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+#pragma unroll
     for (int nt = 0; nt < kH / 16; ++nt) {
-        f4 acc = {0, 0, 0, 0};
-        acc = tile_bits([&](int m, int k) { return s.x[m][k]; },
-                        [&](int k, int n) { return Wbf[oW1 + (nt * 16 + n) * kIn + k]; }, kIn, acc);
+        const f4 acc = tile_rows<kIn, kInLd>(&s.x[0][0], Wbf + oW1 + nt * 16 * kIn, f4{0, 0, 0, 0});
 #pragma unroll
         for (int i = 0; i < 4; ++i) s.h1[4 * g + i][nt * 16 + c] = to_bf16(tanhf(acc[i] + Pf[oB1 + nt * 16 + c]));
     }
     wave_fence();
+#pragma unroll
     for (int nt = 0; nt < kH / 16; ++nt) {
-        f4 acc = {0, 0, 0, 0};
-        acc = tile_bits([&](int m, int k) { return s.h1[m][k]; },
-                        [&](int k, int n) { return Wbf[oW2 + (nt * 16 + n) * kH + k]; }, kH, acc);
+        const f4 acc = tile_rows<kH, kHLd>(&s.h1[0][0], Wbf + oW2 + nt * 16 * kH, f4{0, 0, 0, 0});
 #pragma unroll
         for (int i = 0; i < 4; ++i) s.h2[4 * g + i][nt * 16 + c] = to_bf16(tanhf(acc[i] + Pf[oB2 + nt * 16 + c]));
     }
     wave_fence();
-    f4 acc = {0, 0, 0, 0};
-    acc = tile_bits([&](int m, int k) { return s.h2[m][k]; },
-                    [&](int k, int n) { return Wbf[oW3 + n * kH + k]; }, kH, acc);
+    const f4 acc = tile_rows<kH, kHLd>(&s.h2[0][0], Wbf + oW3, f4{0, 0, 0, 0});
 #pragma unroll
     for (int i = 0; i < 4; ++i) s.y[4 * g + i][c] = acc[i] + Pf[oB3 + c];
     wave_fence();
