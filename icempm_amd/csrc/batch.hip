@@ -57,27 +57,6 @@ static __device__ __forceinline__ void wsync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// Minimum / maximum over the wavefront of the two 16-bit halves of a word:
-// DPP permutations within each row of 16 lanes (pairs, quads, half-row
-// mirror, row mirror), then the four row results through readlane.  Every
-// lane of the wavefront must be active.
-typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));
-template <bool MAX>
-static __device__ __forceinline__ unsigned pk_sel(unsigned a, unsigned b) {
-    const U16x2 x = __builtin_bit_cast(U16x2, a), y = __builtin_bit_cast(U16x2, b);
-    return __builtin_bit_cast(unsigned, MAX ? __builtin_elementwise_max(x, y) : __builtin_elementwise_min(x, y));
-}
-template <bool MAX>
-static __device__ __forceinline__ unsigned wreduce_u16x2(unsigned v) {
-    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
-    const unsigned r0 = (unsigned)__builtin_amdgcn_readlane((int)v, 0), r1 = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
-    const unsigned r2 = (unsigned)__builtin_amdgcn_readlane((int)v, 32), r3 = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
-    return pk_sel<MAX>(pk_sel<MAX>(r0, r1), pk_sel<MAX>(r2, r3));
-}
-
 // Band order of a case's base cells: the rows in pairs (bands, a last odd
 // row alone), a band column by column with the two cells of a column
 // together.  Consecutive cells then cover two rows: a wavefront's 64 points

@@ -387,4 +387,37 @@ static __device__ __forceinline__ int cell_key(double x, double y, double hinv, 
 
 static __device__ __forceinline__ void flag_error(const KParams &k) { k.err[0] = 1; }
 
+// Wavefront reductions through DPP: permutations within each row of 16
+// lanes (pairs, quads, half-row mirror, row mirror), then the four row
+// results through readlane, so the result is wavefront-uniform.  Every lane
+// of the wavefront must be active.
+template <bool MAX>
+static __device__ __forceinline__ int wreduce_i32(int v) {
+    const auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));    // quad_perm 1,0,3,2
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));    // quad_perm 2,3,0,1
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false));   // row_mirror
+    return op(op(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+              op(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// The same for the two 16-bit halves of a word at once.
+typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));
+template <bool MAX>
+static __device__ __forceinline__ unsigned pk_sel(unsigned a, unsigned b) {
+    const U16x2 x = __builtin_bit_cast(U16x2, a), y = __builtin_bit_cast(U16x2, b);
+    return __builtin_bit_cast(unsigned, MAX ? __builtin_elementwise_max(x, y) : __builtin_elementwise_min(x, y));
+}
+template <bool MAX>
+static __device__ __forceinline__ unsigned wreduce_u16x2(unsigned v) {
+    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+    v = pk_sel<MAX>(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
+    const unsigned r0 = (unsigned)__builtin_amdgcn_readlane((int)v, 0), r1 = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+    const unsigned r2 = (unsigned)__builtin_amdgcn_readlane((int)v, 32), r3 = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+    return pk_sel<MAX>(pk_sel<MAX>(r0, r1), pk_sel<MAX>(r2, r3));
+}
+
 }  // namespace icempm
