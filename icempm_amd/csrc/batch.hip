@@ -222,14 +222,15 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         double Jp = 1;
         unsigned char crushed = 0;
         if (valid) {
-            px = S.pos[pt];
-            py = S.pos[pp + pt];
-            F00 = fe0 + (double)R[pp * 2 + pt];
-            F01 = (double)R[pp * 3 + pt];
-            F10 = (double)R[pp * 4 + pt];
-            F11 = fe0 + (double)R[pp * 5 + pt];
-            Jp = (double)R[pp * 10 + pt];
-            crushed = S.crushed[pt];
+            // streamed once per substep: non-temporal, so the grid keeps the L2
+            px = __builtin_nontemporal_load(&S.pos[pt]);
+            py = __builtin_nontemporal_load(&S.pos[pp + pt]);
+            F00 = fe0 + (double)__builtin_nontemporal_load(&R[pp * 2 + pt]);
+            F01 = (double)__builtin_nontemporal_load(&R[pp * 3 + pt]);
+            F10 = (double)__builtin_nontemporal_load(&R[pp * 4 + pt]);
+            F11 = fe0 + (double)__builtin_nontemporal_load(&R[pp * 5 + pt]);
+            Jp = (double)__builtin_nontemporal_load(&R[pp * 10 + pt]);
+            crushed = __builtin_nontemporal_load(&S.crushed[pt]);
         }
         int bx = 0, by = 0;
         double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
@@ -390,13 +391,13 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             fresh = ok && crushed == 0;
         }
         if (ok) {
-            S.pos[pt] = px;
-            S.pos[pp + pt] = py;
+            __builtin_nontemporal_store(px, &S.pos[pt]);
+            __builtin_nontemporal_store(py, &S.pos[pp + pt]);
             if (!defer) {
-                R[pp * 2 + pt] = (V)(F00 - fe0);
-                R[pp * 3 + pt] = (V)F01;
-                R[pp * 4 + pt] = (V)F10;
-                R[pp * 5 + pt] = (V)(F11 - fe0);
+                __builtin_nontemporal_store((V)(F00 - fe0), &R[pp * 2 + pt]);
+                __builtin_nontemporal_store((V)F01, &R[pp * 3 + pt]);
+                __builtin_nontemporal_store((V)F10, &R[pp * 4 + pt]);
+                __builtin_nontemporal_store((V)(F11 - fe0), &R[pp * 5 + pt]);
             }
             // v and Bp: the fused pass uses them at once (a deferred point's
             // go to the map kernel in its queue entry) and the next G2P
