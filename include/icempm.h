@@ -220,8 +220,9 @@ int icempm_batch_last_ms(const icempm_batch *b, float *ms);
 int icempm_batch_num_points(const icempm_batch *b);
 long icempm_batch_num_nodes(const icempm_batch *b);
 /* Device timing of the fused particle pass: with stride > 0, every
- * stride-th fused launch of a stream run (use_graph = 0; a graph run then
- * fails) is bracketed by events and the durations are summed; stride 0
+ * stride-th fused particle kernel of a stream run (use_graph = 0; a graph
+ * run then fails) is bracketed by events and the durations are summed (the
+ * learned map's kernel that follows it is outside the bracket); stride 0
  * switches it off.  Enabling resets the sums. */
 int icempm_batch_kernel_timing(icempm_batch *b, int stride);
 int icempm_batch_kernel_timing_read(icempm_batch *b, double *total_ms, int *launches);
