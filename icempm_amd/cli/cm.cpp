@@ -8,22 +8,212 @@
 // end time or an out-of-grid point.
 //
 // usage: cm <config.json> [--snapshots DIR] [--ordered] [--pq] [--cycles N]
+//        cm --batch <manifest.json> [--out DIR] [--f32] [--cycles N]
+//
+// The batch mode (no counterpart in the reference) runs many configurations
+// as one batch on the batched engine: the manifest is a JSON object whose
+// "cases" array lists configuration files (paths relative to the manifest),
+// with optional "precision" ("f32" or "f64") and "output" (directory).
+// Each configuration's raw point file is ingested as the single-case path
+// does (LoadRawPoints); every case must have a uniform material
+// (GrainVariability 0) and all share one time step.  Output cycles are
+// UpdateEveryNthStep substeps of the first case; the run ends after
+// --cycles cycles or when the first case reaches its SimulationEndTime.
+// Each cycle appends every case's per-substep total indenter force to
+// DIR/case_K.csv (substep, time, fx, fy) and one line to DIR/batch.csv.
 #include <atomic>
+#include <cmath>
 #include <cstdlib>
 #include <filesystem>
+#include <fstream>
 #include <iostream>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include "icempm.h"
 #include "model.h"
 #include "snapshotmanager.h"
 
+namespace {
+
+const char *kUsage =
+    "usage: cm <config.json> [--snapshots DIR] [--ordered] [--pq] [--cycles N]\n"
+    "       cm --batch <manifest.json> [--out DIR] [--f32] [--cycles N]\n";
+
+// The "cases" string array and the "precision" / "output" strings of a
+// manifest (a JSON object; other keys are ignored).
+struct Manifest {
+    std::vector<std::string> cases;
+    std::string precision = "f64", output;
+};
+
+bool read_string(const std::string &s, size_t &i, std::string &out) {
+    if (i >= s.size() || s[i] != '"') return false;
+    out.clear();
+    for (++i; i < s.size() && s[i] != '"'; ++i) {
+        if (s[i] == '\\' && i + 1 < s.size()) ++i;
+        out.push_back(s[i]);
+    }
+    if (i >= s.size()) return false;
+    ++i;
+    return true;
+}
+
+void skip_ws(const std::string &s, size_t &i) {
+    while (i < s.size() && std::isspace((unsigned char)s[i])) ++i;
+}
+
+Manifest parse_manifest(const std::string &path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("manifest " + path + " not found");
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string s = ss.str();
+    Manifest m;
+    size_t i = 0;
+    skip_ws(s, i);
+    if (i >= s.size() || s[i] != '{') throw std::runtime_error("manifest is not a JSON object");
+    ++i;
+    while (true) {
+        skip_ws(s, i);
+        if (i < s.size() && s[i] == '}') break;
+        std::string key;
+        if (!read_string(s, i, key)) throw std::runtime_error("manifest: expected a key");
+        skip_ws(s, i);
+        if (i >= s.size() || s[i] != ':') throw std::runtime_error("manifest: expected ':'");
+        ++i;
+        skip_ws(s, i);
+        if (key == "cases") {
+            if (i >= s.size() || s[i] != '[') throw std::runtime_error("manifest: \"cases\" must be an array");
+            ++i;
+            while (true) {
+                skip_ws(s, i);
+                if (i < s.size() && s[i] == ']') {
+                    ++i;
+                    break;
+                }
+                std::string c;
+                if (!read_string(s, i, c)) throw std::runtime_error("manifest: a case must be a string");
+                m.cases.push_back(c);
+                skip_ws(s, i);
+                if (i < s.size() && s[i] == ',') ++i;
+            }
+        } else if (i < s.size() && s[i] == '"') {
+            std::string v;
+            if (!read_string(s, i, v)) throw std::runtime_error("manifest: bad string");
+            if (key == "precision") m.precision = v;
+            if (key == "output") m.output = v;
+        } else {   // a number or literal: skip to the next ',' or '}'
+            while (i < s.size() && s[i] != ',' && s[i] != '}') ++i;
+        }
+        skip_ws(s, i);
+        if (i < s.size() && s[i] == ',') {
+            ++i;
+            continue;
+        }
+        if (i < s.size() && s[i] == '}') break;
+        throw std::runtime_error("manifest: expected ',' or '}'");
+    }
+    if (m.cases.empty()) throw std::runtime_error("manifest: no cases");
+    return m;
+}
+
+void check(icempm_batch *b, int rc, const char *what) {
+    if (rc != 0)
+        throw std::runtime_error(std::string(what) + ": " + (b ? icempm_batch_last_error(b) : "") +
+                                 " (code " + std::to_string(rc) + ")");
+}
+
+int run_batch(const std::string &manifestPath, std::string out, bool f32, long maxCycles) {
+    const Manifest man = parse_manifest(manifestPath);
+    if (out.empty()) out = man.output.empty() ? "cm_batch" : man.output;
+    const bool useF32 = f32 || man.precision == "f32";
+    const std::filesystem::path base = std::filesystem::path(manifestPath).parent_path();
+    icempm_batch *b = nullptr;
+    if (icempm_batch_create(&b, useF32 ? ICEMPM_BATCH_F32 : ICEMPM_BATCH_F64) != 0 || !b)
+        throw std::runtime_error("batch_create failed");
+    struct Guard {
+        icempm_batch *b;
+        ~Guard() { icempm_batch_destroy(b); }
+    } guard{b};
+    std::vector<icy::SimParams> prms;
+    long total = 0;
+    for (const std::string &c : man.cases) {
+        const std::filesystem::path cfg = std::filesystem::path(c).is_absolute() ? std::filesystem::path(c) : base / c;
+        // the single-case ingest: parse, load and sort the raw points on the device, read them back
+        icy::Model model;
+        icy::SnapshotManager snapshot;
+        model.gpu.initialize();
+        model.Reset();
+        snapshot.model = &model;
+        const std::string raw = model.prms.ParseFile(cfg.string());
+        snapshot.LoadRawPoints(raw);
+        std::vector<double> soa(16 * model.prms.nPtsPitch);
+        if (icempm_read_points(model.gpu.context(), soa.data()) != 0)
+            throw std::runtime_error("reading the ingested points of " + cfg.string() + " failed");
+        const int k = icempm_batch_add_case(b, &model.prms, sizeof(icy::SimParams), soa.data(), model.prms.nPtsPitch);
+        if (k < 0) check(b, -k, ("add_case " + cfg.string()).c_str());
+        prms.push_back(model.prms);
+        total += model.prms.nPts;
+        std::cerr << "cm: case " << k << " " << cfg.string() << ": " << model.prms.nPts << " points, grid "
+                  << model.prms.GridX << " x " << model.prms.GridY << "\n";
+    }
+    check(b, icempm_batch_finalize(b), "finalize");
+    const int N = prms[0].UpdateEveryNthStep;
+    const double dt = prms[0].InitialTimeStep;
+    if (N <= 0) throw std::runtime_error("UpdateEveryNthStep must be positive");
+    long cycles = maxCycles;
+    if (cycles < 0) cycles = (long)std::ceil(prms[0].SimulationEndTime / (dt * N));
+    std::filesystem::create_directories(out);
+    std::vector<std::ofstream> csv;
+    for (size_t c = 0; c < prms.size(); ++c) {
+        csv.emplace_back(std::filesystem::path(out) / ("case_" + std::to_string(c) + ".csv"));
+        csv.back() << "substep,time,fx,fy\n";
+        csv.back().precision(17);
+    }
+    std::ofstream summary(std::filesystem::path(out) / "batch.csv");
+    summary << "cycle,substeps,ms,particle_updates_per_s\n";
+    std::cerr << "cm: batch of " << prms.size() << " cases, " << total << " points, " << (useF32 ? "f32" : "f64")
+              << " storage, " << cycles << " cycles of " << N << " substeps\n";
+    std::vector<double> hist(2 * (size_t)N);
+    double t = 0;   // by repeated addition, as Model::Step does
+    long step = 0;
+    for (long cy = 1; cy <= cycles; ++cy) {
+        check(b, icempm_batch_run(b, N, 1), "run");
+        float ms = 0;
+        check(b, icempm_batch_last_ms(b, &ms), "last_ms");
+        int flag = 0;
+        check(b, icempm_batch_error_flag(b, &flag), "error_flag");
+        std::vector<double> times(N);
+        for (int i = 0; i < N; ++i) {
+            t += dt;
+            times[i] = t;
+        }
+        for (size_t c = 0; c < prms.size(); ++c) {
+            check(b, icempm_batch_force_history(b, (int)c, hist.data(), N), "force_history");
+            for (int i = 0; i < N; ++i)
+                csv[c] << step + i + 1 << "," << times[i] << "," << hist[2 * i] << "," << hist[2 * i + 1] << "\n";
+        }
+        step += N;
+        summary << cy << "," << N << "," << ms << "," << (ms > 0 ? total * (double)N / (ms * 1e-3) : 0.0) << "\n";
+        std::cerr << "cm: cycle " << cy << " of " << cycles << ", " << ms / N << " ms per substep\n";
+        if (flag) {
+            std::cerr << "cm: a point left the grid\n";
+            return 3;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
 int main(int argc, char **argv)
 {
-    std::string config, dir = "cm_snapshots";
-    bool ordered = false, pq = false;
+    std::string config, dir = "cm_snapshots", manifest, out;
+    bool ordered = false, pq = false, f32 = false;
     long maxCycles = -1;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -31,14 +221,32 @@ int main(int argc, char **argv)
         else if (a == "--ordered") ordered = true;
         else if (a == "--pq") pq = true;
         else if (a == "--cycles" && i + 1 < argc) maxCycles = std::atol(argv[++i]);
+        else if (a == "--batch" && i + 1 < argc) manifest = argv[++i];
+        else if (a == "--out" && i + 1 < argc) out = argv[++i];
+        else if (a == "--f32") f32 = true;
         else if (!a.empty() && a[0] != '-' && config.empty()) config = a;
         else {
-            std::cerr << "usage: cm <config.json> [--snapshots DIR] [--ordered] [--pq] [--cycles N]\n";
+            std::cerr << kUsage;
             return 2;
         }
     }
+    if (!manifest.empty()) {
+        if (!config.empty() || ordered || pq) {
+            std::cerr << kUsage;
+            return 2;
+        }
+        try {
+            const int rc = run_batch(manifest, out, f32, maxCycles);
+            if (rc) return rc;
+        } catch (const std::exception &e) {
+            std::cerr << "cm: " << e.what() << "\n";
+            return 1;
+        }
+        std::cout << "cm done\n";
+        return 0;
+    }
     if (config.empty()) {
-        std::cerr << "usage: cm <config.json> [--snapshots DIR] [--ordered] [--pq] [--cycles N]\n";
+        std::cerr << kUsage;
         return 2;
     }
 

@@ -116,7 +116,7 @@ struct MapEntry {
     int pt, cs;
     double F[4];
     double Jp;
-    double x[2], v[2], B[4];   // position, velocity and Bp for its P2G (v, Bp at storage precision)
+    double x[2], v[2], B[4];   // position, velocity and Bp for its P2G
 };
 
 // Queue of deferred points of one pass (learned map only).
@@ -372,9 +372,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                         e.F[0] = F00; e.F[1] = F01; e.F[2] = F10; e.F[3] = F11;
                         e.Jp = Jp;
                         e.x[0] = px; e.x[1] = py;
-                        e.v[0] = (double)(V)vx; e.v[1] = (double)(V)vy;
-                        e.B[0] = (double)(V)B00; e.B[1] = (double)(V)B01;
-                        e.B[2] = (double)(V)B10; e.B[3] = (double)(V)B11;
+                        e.v[0] = vx; e.v[1] = vy;
+                        e.B[0] = B00; e.B[1] = B01; e.B[2] = B10; e.B[3] = B11;
                     } else {
                         flag_error(k);
                     }
@@ -414,11 +413,11 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             if (Jp != Jp0 && !defer) R[pp * 10 + pt] = (V)Jp;
             if (crushed != crushed0) S.crushed[pt] = crushed;
             if constexpr (F32) {
-                // continue the P2G from the stored precision, as a separate pass would
-                vx = (double)(V)vx; vy = (double)(V)vy;
+                // Fe is what the next substep reads back: the P2G takes it at
+                // its stored precision.  v and Bp stay FP64: a fused pass does
+                // not store them, and the next G2P recomputes them.
                 F00 = fe0 + (double)(V)(F00 - fe0); F01 = (double)(V)F01;
                 F10 = (double)(V)F10; F11 = fe0 + (double)(V)(F11 - fe0);
-                B00 = (double)(V)B00; B01 = (double)(V)B01; B10 = (double)(V)B10; B11 = (double)(V)B11;
             }
         }
     }

@@ -61,3 +61,85 @@ def test_cm_writes_oracle_snapshots(gpu_available, tmp_path):
     dev = binary(snap, "/Points", "<f8", tmp_path).reshape(16, -1)
     assert np.array_equal(dev.view(np.uint64), pts.view(np.uint64))
     assert np.array_equal(binary(snap, "/Indenter_2D", "<f8", tmp_path), bins)
+
+
+def case_config(case, raw):
+    p = SimParams()
+    configure_params(p, case)
+    return p, {
+        "InitialTimeStep": case.dt, "YoungsModulus": case.youngs, "GridX": case.grid_x,
+        "GridY": case.grid_y, "GridXDimension": p.GridXDimension, "SimulationEndTime": 1.0,
+        "PoissonsRatio": case.poisson, "Gravity": case.gravity, "Density": case.density,
+        "IndDiameter": case.ind_diameter, "IndVelocity": case.ind_velocity, "IndDepth": case.ind_depth,
+        "IceCompressiveStrength": case.compressive, "IceTensileStrength": case.tensile,
+        "IceShearStrength": case.shear, "DP_phi": case.dp_phi_deg, "DP_threshold_p": case.dp_threshold,
+        "GrainVariability": case.grain_variability, "InputRawPoints": str(raw),
+    }
+
+
+def run_cm_batch(tmp_path, cases, f32):
+    names, refs = [], []
+    for k, case in enumerate(cases):
+        xy, g = lattice_points(case)
+        raw = tmp_path / f"raw{k}.h5"
+        write_raw_points(str(raw), xy, g, case.volume, 7)
+        p, cfg = case_config(case, raw)
+        (tmp_path / f"c{k}.json").write_text(json.dumps(cfg))
+        names.append(f"c{k}.json")
+        refs.append((p, xy, g, case.volume))
+    (tmp_path / "m.json").write_text(json.dumps({"cases": names, "precision": "f32" if f32 else "f64"}))
+    out = tmp_path / "out"
+    r = subprocess.run([CLI_BIN, "--batch", str(tmp_path / "m.json"), "--out", str(out), "--cycles", "1"],
+                       stdin=subprocess.DEVNULL, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=180)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0 and "cm done" in r.stdout
+    N = refs[0][0].UpdateEveryNthStep
+    summary = np.loadtxt(out / "batch.csv", delimiter=",", skiprows=1, ndmin=2)
+    assert summary.shape == (1, 4) and summary[0, 1] == N and summary[0, 3] > 0
+    hists = []
+    for k in range(len(cases)):
+        rows = np.loadtxt(out / f"case_{k}.csv", delimiter=",", skiprows=1)
+        assert rows.shape == (N, 4) and np.array_equal(rows[:, 0], np.arange(1, N + 1))
+        hists.append(rows[:, 2:])
+    return refs, N, hists
+
+
+def test_cm_batch_manifest_matches_oracle(gpu_available, tmp_path):
+    """cm --batch: configurations listed in a manifest run as one batch (FP64
+    storage); each case's per-substep indenter force over one output cycle
+    (case_K.csv) equals the oracle run of that case alone within 1e-6."""
+    from icempm_amd.batch import batch_cases
+    print("cli: cm --batch, 3 cases, 1 cycle f64", flush=True)
+    refs, N, hists = run_cm_batch(tmp_path, batch_cases(3, 1500, 4000, seed=21), False)
+    for k, ((p, xy, g, vol), h) in enumerate(zip(refs, hists)):
+        pts = oracle.ingest(p, xy, g, vol, (xy.shape[0] + 63) // 64 * 64)
+        p.nGridPitch = (p.GridTotal + 63) // 64 * 64
+        grid = oracle.new_grid(p)
+        bins = np.zeros(2 * p.n_indenter_subdivisions)
+        href, err = oracle.run(p, pts, grid, bins, N)
+        assert err == 0
+        e = np.linalg.norm(h - href) / np.linalg.norm(href)
+        print(f"case {k}: force rel L2 {e:.2e}", flush=True)
+        assert np.abs(href).sum() > 0 and e <= 1e-6, (k, e)
+
+
+def test_cm_batch_manifest_f32_matches_library(gpu_available, tmp_path):
+    """cm --batch with FP32 storage gives the library's FP32 batch run of the
+    same cases (icempm_amd.batch) to the run-to-run reproducibility of the
+    batched pass (hardware atomics; 1e-4 over the 2,500-substep cycle).  The
+    FP32 storage error itself is held to the oracle in test_gpu_batch."""
+    from icempm_amd._lib import BATCH_F32
+    from icempm_amd.batch import batch_cases, build_batch
+    print("cli: cm --batch, 3 cases, 1 cycle f32", flush=True)
+    cases = batch_cases(3, 1500, 4000, seed=21)
+    _, N, hists = run_cm_batch(tmp_path, cases, True)
+    b = build_batch(cases, BATCH_F32)
+    b.run(N)
+    assert b.error_flag() == 0
+    for k, h in enumerate(hists):
+        ref = b.force_history(k, N)
+        e = np.linalg.norm(h - ref) / np.linalg.norm(ref)
+        print(f"case {k}: force rel L2 vs library {e:.2e}", flush=True)
+        assert e <= 1e-4, (k, e)
+    b.close()
