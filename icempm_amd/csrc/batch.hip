@@ -38,19 +38,39 @@ struct CaseParams {
     double dt, vol, mass, h, h_inv, Dinv, mu, kappa, gravity, indRsq, indV;
     double ics, its, iss, tanPhi, dpThr, x0, y0;
     double pmax, msq, betaPmax, onePlus2Beta, slope;   // NACC constants of the case's material
+    double posQ, posInvQ;   // FP32 storage: position quantum (grid width / 2^32) and its inverse
     int gx, gy, gridTotal, nPts, ptOffset, gridOffset, pad0, pad1;
 };
 
 constexpr int kWX = 48, kWS = 52, kWY = 4;   // window width, padded row stride, rows
 
+// Point rows.  FP64 storage: positions FP64.  FP32 storage: positions as
+// 32-bit fixed point over the case's grid width (a quantum of gx h / 2^32,
+// 7e-10 m for a 3 m grid: a substep moves a point by ~1e-6 m, which an FP32
+// coordinate could not resolve), the other values FP32.
 template <bool F32>
 struct Rows {
     using V = typename std::conditional<F32, float, double>::type;
-    double *pos;       // 2 rows
+    using P = typename std::conditional<F32, unsigned int, double>::type;
+    P *pos;            // 2 rows
     V *rest;           // 11 rows: vel(2) Fe(4) Bp(4) Jp(1)
     unsigned char *crushed;
     size_t pitch;
 };
+
+// a stored coordinate as FP64, and back (round to nearest; outside the grid
+// and NaN go to the grid's edge, where the next stencil check flags them)
+static __host__ __device__ inline double pos_get(double v, const CaseParams &) { return v; }
+static __host__ __device__ inline double pos_get(unsigned int v, const CaseParams &cp) { return (double)v * cp.posQ; }
+template <class P>
+static __host__ __device__ inline P pos_put(double x, const CaseParams &cp);
+template <>
+__host__ __device__ inline double pos_put<double>(double x, const CaseParams &) { return x; }
+template <>
+__host__ __device__ inline unsigned int pos_put<unsigned int>(double x, const CaseParams &cp) {
+    const double u = rint(x * cp.posInvQ);
+    return (u >= 0.0 && u <= 4294967295.0) ? (unsigned int)u : (u < 0.0 ? 0u : 4294967295u);
+}
 
 static __device__ __forceinline__ void wsync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -184,7 +204,8 @@ constexpr int kWaveWinBytes = 3 * kWY * kWS * 8;
 // and the case's parameters arrive meanwhile.
 template <bool F32, bool G2P, bool P2G, int RMAP>
 __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict__ chunks, int nChunks,
-                                                         const CaseParams *__restrict__ cases, double *pos,
+                                                         const CaseParams *__restrict__ cases,
+                                                         typename Rows<F32>::P *pos,
                                                          typename Rows<F32>::V *rest, unsigned char *crushedRow,
                                                          size_t pitch, const double *__restrict__ gridV,
                                                          double *__restrict__ gridOut, size_t gpitch,
@@ -223,8 +244,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         unsigned char crushed = 0;
         if (valid) {
             // streamed once per substep: non-temporal, so the grid keeps the L2
-            px = __builtin_nontemporal_load(&S.pos[pt]);
-            py = __builtin_nontemporal_load(&S.pos[pp + pt]);
+            px = pos_get(__builtin_nontemporal_load(&S.pos[pt]), cp);
+            py = pos_get(__builtin_nontemporal_load(&S.pos[pp + pt]), cp);
             F00 = fe0 + (double)__builtin_nontemporal_load(&R[pp * 2 + pt]);
             F01 = (double)__builtin_nontemporal_load(&R[pp * 3 + pt]);
             F10 = (double)__builtin_nontemporal_load(&R[pp * 4 + pt]);
@@ -371,7 +392,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                         e.cs = ch.z;
                         e.F[0] = F00; e.F[1] = F01; e.F[2] = F10; e.F[3] = F11;
                         e.Jp = Jp;
-                        e.x[0] = px; e.x[1] = py;
+                        // the position at its stored precision, as the P2G here would take it
+                        using PT = typename Rows<F32>::P;
+                        e.x[0] = pos_get(pos_put<PT>(px, cp), cp);
+                        e.x[1] = pos_get(pos_put<PT>(py, cp), cp);
                         e.v[0] = vx; e.v[1] = vy;
                         e.B[0] = B00; e.B[1] = B01; e.B[2] = B10; e.B[3] = B11;
                     } else {
@@ -390,8 +414,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             fresh = ok && crushed == 0;
         }
         if (ok) {
-            __builtin_nontemporal_store(px, &S.pos[pt]);
-            __builtin_nontemporal_store(py, &S.pos[pp + pt]);
+            using PT = typename Rows<F32>::P;
+            const PT qx = pos_put<PT>(px, cp), qy = pos_put<PT>(py, cp);
+            __builtin_nontemporal_store(qx, &S.pos[pt]);
+            __builtin_nontemporal_store(qy, &S.pos[pp + pt]);
             if (!defer) {
                 __builtin_nontemporal_store((V)(F00 - fe0), &R[pp * 2 + pt]);
                 __builtin_nontemporal_store((V)F01, &R[pp * 3 + pt]);
@@ -413,9 +439,11 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             if (Jp != Jp0 && !defer) R[pp * 10 + pt] = (V)Jp;
             if (crushed != crushed0) S.crushed[pt] = crushed;
             if constexpr (F32) {
-                // Fe is what the next substep reads back: the P2G takes it at
-                // its stored precision.  v and Bp stay FP64: a fused pass does
-                // not store them, and the next G2P recomputes them.
+                // position and Fe are what the next substep reads back: the
+                // P2G takes them at their stored precision.  v and Bp stay
+                // FP64: a fused pass does not store them, and the next G2P
+                // recomputes them.
+                px = pos_get(qx, cp); py = pos_get(qy, cp);
                 F00 = fe0 + (double)(V)(F00 - fe0); F01 = (double)(V)F01;
                 F10 = (double)(V)F10; F11 = fe0 + (double)(V)(F11 - fe0);
             }
@@ -426,8 +454,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         bool have = valid;
         if constexpr (!G2P) {
             if (valid) {
-                px = S.pos[pt];
-                py = S.pos[pp + pt];
+                px = pos_get(S.pos[pt], cp);
+                py = pos_get(S.pos[pp + pt], cp);
                 vx = (double)R[pt];
                 vy = (double)R[pp + pt];
                 F00 = fe0 + (double)R[pp * 2 + pt];
@@ -792,7 +820,8 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
 
 // Cell key of every point of the batch, offset by its case's first node so
 // that a sort keeps the cases contiguous and in order; vals = slot index.
-__global__ void k_batch_keys(const double *__restrict__ pos, size_t pitch, int nPts,
+template <class PT>
+__global__ void k_batch_keys(const PT *__restrict__ pos, size_t pitch, int nPts,
                              const CaseParams *__restrict__ cases, int nCases, int *__restrict__ keys,
                              int *__restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -803,7 +832,7 @@ __global__ void k_batch_keys(const double *__restrict__ pos, size_t pitch, int n
         if (cases[mid].ptOffset <= i) lo = mid; else hi = mid - 1;
     }
     const CaseParams &cp = cases[lo];
-    const double x = pos[i], y = pos[pitch + i];
+    const double x = pos_get(pos[i], cp), y = pos_get(pos[pitch + i], cp);
     int key = cp.gridTotal - 1;
     if (x == x && y == y) {
         const double bx = fmin(fmax(x * cp.h_inv - 0.5, 0.0), (double)(cp.gx - 1));
@@ -851,7 +880,7 @@ struct icempm_batch {
     std::vector<HostCase> hc;
 
     // device
-    double *pos = nullptr;
+    void *pos = nullptr;          // 2 rows of Rows<F32>::P
     void *rest = nullptr;
     unsigned char *crushed = nullptr;
     size_t pitch = 0;
@@ -902,13 +931,13 @@ struct icempm_batch {
     int *orig = nullptr;                    // batch index of the point in each slot
     // re-sort scratch
     int *sKeys = nullptr, *sVals = nullptr, *sKeys2 = nullptr, *sVals2 = nullptr, *sTmp = nullptr;
-    double *tpos = nullptr;
+    void *tpos = nullptr;
     void *trest = nullptr;
     unsigned char *tcrushed = nullptr;
     int *torig = nullptr;
     // checkpoint of the state
     bool haveCheckpoint = false;
-    double *kpos = nullptr;
+    void *kpos = nullptr;
     void *krest = nullptr;
     unsigned char *kcrushed = nullptr;
     int *korig = nullptr;
@@ -951,7 +980,8 @@ constexpr int kMapBlocks = 256;
 template <bool F32, bool G2P, bool P2G, int RMAP>
 void launch_one(icempm_batch *b, unsigned nb, const double *gV, double *gO, const LearnArgs &la,
                 const MapQueue &mq) {
-    Rows<F32> S{b->pos, reinterpret_cast<typename Rows<F32>::V *>(b->rest), b->crushed, b->pitch};
+    Rows<F32> S{static_cast<typename Rows<F32>::P *>(b->pos), reinterpret_cast<typename Rows<F32>::V *>(b->rest),
+                b->crushed, b->pitch};
     hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream,
                        b->chunks, b->nChunks, b->cases, S.pos, S.rest, S.crushed, S.pitch, gV, gO, b->gpitch,
                        b->err, la, mq);
@@ -1080,8 +1110,14 @@ int current_keys(icempm_batch *b, std::vector<int> &local) {
         BCHK(b, hipMalloc(&b->sKeys, (size_t)n * sizeof(int)), "keys");
         BCHK(b, hipMalloc(&b->sVals, (size_t)n * sizeof(int)), "keys");
     }
-    hipLaunchKernelGGL(k_batch_keys, dim3((n + 255) / 256), dim3(256), 0, b->stream, b->pos, b->pitch, n,
-                       b->cases, (int)b->hc.size(), b->sKeys, b->sVals);
+    if (b->precision == ICEMPM_BATCH_F32)
+        hipLaunchKernelGGL(k_batch_keys<unsigned int>, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                           static_cast<const unsigned int *>(b->pos), b->pitch, n, b->cases, (int)b->hc.size(),
+                           b->sKeys, b->sVals);
+    else
+        hipLaunchKernelGGL(k_batch_keys<double>, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                           static_cast<const double *>(b->pos), b->pitch, n, b->cases, (int)b->hc.size(), b->sKeys,
+                           b->sVals);
     BCHK(b, hipGetLastError(), "keys");
     local.resize((size_t)n);
     BCHK(b, hipMemcpyAsync(local.data(), b->sKeys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->stream), "keys");
@@ -1091,6 +1127,10 @@ int current_keys(icempm_batch *b, std::vector<int> &local) {
         for (int i = 0; i < cp.nPts; ++i) local[(size_t)cp.ptOffset + i] -= cp.gridOffset;
     }
     return ICEMPM_OK;
+}
+
+size_t pos_bytes(const icempm_batch *b) {
+    return 2 * b->pitch * (b->precision == ICEMPM_BATCH_F32 ? sizeof(unsigned int) : sizeof(double));
 }
 
 size_t rest_bytes(const icempm_batch *b) {
@@ -1110,7 +1150,7 @@ int resort(icempm_batch *b) {
         BCHK(b, hipMalloc(&b->sKeys2, (size_t)n * sizeof(int)), "re-sort");
         BCHK(b, hipMalloc(&b->sVals2, (size_t)n * sizeof(int)), "re-sort");
         BCHK(b, hipMalloc(&b->sTmp, radix_tmp_ints((size_t)n) * sizeof(int)), "re-sort");
-        BCHK(b, hipMalloc(&b->tpos, 2 * b->pitch * sizeof(double)), "re-sort");
+        BCHK(b, hipMalloc(&b->tpos, pos_bytes(b)), "re-sort");
         BCHK(b, hipMalloc(&b->trest, rest_bytes(b)), "re-sort");
         BCHK(b, hipMalloc(&b->tcrushed, b->pitch), "re-sort");
         BCHK(b, hipMalloc(&b->torig, b->pitch * sizeof(int)), "re-sort");
@@ -1118,12 +1158,12 @@ int resort(icempm_batch *b) {
     BCHK(b, radix_sort_pairs(b->sKeys, b->sVals, b->sKeys2, b->sVals2, (size_t)n, b->sTmp, b->stream), "re-sort");
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (b->precision == ICEMPM_BATCH_F32) {
-        Rows<true> a{b->pos, reinterpret_cast<float *>(b->rest), b->crushed, b->pitch};
-        Rows<true> t{b->tpos, reinterpret_cast<float *>(b->trest), b->tcrushed, b->pitch};
+        Rows<true> a{static_cast<unsigned int *>(b->pos), reinterpret_cast<float *>(b->rest), b->crushed, b->pitch};
+        Rows<true> t{static_cast<unsigned int *>(b->tpos), reinterpret_cast<float *>(b->trest), b->tcrushed, b->pitch};
         hipLaunchKernelGGL(k_batch_permute<true>, dim3(nb), dim3(256), 0, b->stream, a, t, b->orig, b->torig, b->sVals, n);
     } else {
-        Rows<false> a{b->pos, reinterpret_cast<double *>(b->rest), b->crushed, b->pitch};
-        Rows<false> t{b->tpos, reinterpret_cast<double *>(b->trest), b->tcrushed, b->pitch};
+        Rows<false> a{static_cast<double *>(b->pos), reinterpret_cast<double *>(b->rest), b->crushed, b->pitch};
+        Rows<false> t{static_cast<double *>(b->tpos), reinterpret_cast<double *>(b->trest), b->tcrushed, b->pitch};
         hipLaunchKernelGGL(k_batch_permute<false>, dim3(nb), dim3(256), 0, b->stream, a, t, b->orig, b->torig, b->sVals, n);
     }
     BCHK(b, hipGetLastError(), "re-sort");
@@ -1271,6 +1311,8 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
         c.slope = -(2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin)) / -3e6;
     }
     c.gx = P.GridX; c.gy = P.GridY; c.gridTotal = P.GridTotal; c.nPts = P.nPts;
+    c.posQ = (c.gx * c.h) / 4294967296.0;   // FP32 storage: positions in [0, gx h) on 32 bits
+    c.posInvQ = 1.0 / c.posQ;
     const int n = P.nPts;
     h.pos.resize(2 * (size_t)n);
     h.rest.resize(11 * (size_t)n);
@@ -1328,7 +1370,7 @@ int icempm_batch_finalize(icempm_batch *b) {
     b->gpitch = ((size_t)gofs + 63) / 64 * 64;
     chunk_table(b, keys, b->hcstart, b->hccase);
     const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
-    BCHK(b, hipMalloc(&b->pos, 2 * b->pitch * sizeof(double)), "batch finalize");
+    BCHK(b, hipMalloc(&b->pos, pos_bytes(b)), "batch finalize");
     BCHK(b, hipMalloc(&b->rest, 11 * b->pitch * vs), "batch finalize");
     BCHK(b, hipMalloc(&b->crushed, b->pitch), "batch finalize");
     BCHK(b, hipMalloc(&b->orig, b->pitch * sizeof(int)), "batch finalize");
@@ -1366,7 +1408,20 @@ int icempm_batch_finalize(icempm_batch *b) {
         }
         hcp.push_back(h.cp);
     }
-    BCHK(b, hipMemcpy(b->pos, hpos.data(), hpos.size() * sizeof(double), hipMemcpyHostToDevice), "batch upload");
+    if (b->precision == ICEMPM_BATCH_F32) {
+        std::vector<unsigned int> q(hpos.size(), 0u);
+        for (size_t ci = 0; ci < hcp.size(); ++ci) {
+            const CaseParams &cp = hcp[ci];
+            for (int k = 0; k < cp.nPts; ++k)
+                for (int a = 0; a < 2; ++a) {
+                    const size_t i = (size_t)a * b->pitch + cp.ptOffset + k;
+                    q[i] = pos_put<unsigned int>(hpos[i], cp);
+                }
+        }
+        BCHK(b, hipMemcpy(b->pos, q.data(), q.size() * sizeof(unsigned int), hipMemcpyHostToDevice), "batch upload");
+    } else {
+        BCHK(b, hipMemcpy(b->pos, hpos.data(), hpos.size() * sizeof(double), hipMemcpyHostToDevice), "batch upload");
+    }
     BCHK(b, hipMemcpy(b->rest, hrest.data(), hrest.size(), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->crushed, hcr.data(), hcr.size(), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->orig, horig.data(), horig.size() * sizeof(int), hipMemcpyHostToDevice), "batch upload");
@@ -1527,8 +1582,19 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
     const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
     std::vector<double> hpos(2 * (size_t)n);
     std::vector<unsigned char> hrest(11 * (size_t)n * vs), hcr(n);
-    BCHK(b, hipMemcpy(hpos.data(), b->pos + o, n * sizeof(double), hipMemcpyDeviceToHost), "read_points");
-    BCHK(b, hipMemcpy(hpos.data() + n, b->pos + b->pitch + o, n * sizeof(double), hipMemcpyDeviceToHost), "read_points");
+    if (b->precision == ICEMPM_BATCH_F32) {
+        std::vector<unsigned int> q(2 * (size_t)n);
+        const unsigned int *P = static_cast<const unsigned int *>(b->pos);
+        BCHK(b, hipMemcpy(q.data(), P + o, n * sizeof(unsigned int), hipMemcpyDeviceToHost), "read_points");
+        BCHK(b, hipMemcpy(q.data() + n, P + b->pitch + o, n * sizeof(unsigned int), hipMemcpyDeviceToHost),
+             "read_points");
+        for (size_t i = 0; i < q.size(); ++i) hpos[i] = pos_get(q[i], cp);
+    } else {
+        const double *P = static_cast<const double *>(b->pos);
+        BCHK(b, hipMemcpy(hpos.data(), P + o, n * sizeof(double), hipMemcpyDeviceToHost), "read_points");
+        BCHK(b, hipMemcpy(hpos.data() + n, P + b->pitch + o, n * sizeof(double), hipMemcpyDeviceToHost),
+             "read_points");
+    }
     for (int r = 0; r < 11; ++r)
         BCHK(b, hipMemcpy(hrest.data() + (size_t)r * n * vs, (unsigned char *)b->rest + ((size_t)r * b->pitch + o) * vs,
                           n * vs, hipMemcpyDeviceToHost), "read_points");
@@ -1681,12 +1747,12 @@ int icempm_batch_checkpoint(icempm_batch *b) {
     if (!b) return ICEMPM_E_ARG;
     if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "checkpoint: batch not finalized");
     if (!b->kpos) {
-        BCHK(b, hipMalloc(&b->kpos, 2 * b->pitch * sizeof(double)), "checkpoint");
+        BCHK(b, hipMalloc(&b->kpos, pos_bytes(b)), "checkpoint");
         BCHK(b, hipMalloc(&b->krest, rest_bytes(b)), "checkpoint");
         BCHK(b, hipMalloc(&b->kcrushed, b->pitch), "checkpoint");
         BCHK(b, hipMalloc(&b->korig, b->pitch * sizeof(int)), "checkpoint");
     }
-    BCHK(b, hipMemcpyAsync(b->kpos, b->pos, 2 * b->pitch * sizeof(double), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+    BCHK(b, hipMemcpyAsync(b->kpos, b->pos, pos_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
     BCHK(b, hipMemcpyAsync(b->krest, b->rest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
     BCHK(b, hipMemcpyAsync(b->kcrushed, b->crushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "checkpoint");
     BCHK(b, hipMemcpyAsync(b->korig, b->orig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
@@ -1704,7 +1770,7 @@ int icempm_batch_restore(icempm_batch *b) {
     if (!b) return ICEMPM_E_ARG;
     if (!b->haveCheckpoint) return bfail(b, ICEMPM_E_STATE, "restore: no checkpoint");
     BCHK(b, hipStreamSynchronize(b->stream), "restore");
-    BCHK(b, hipMemcpyAsync(b->pos, b->kpos, 2 * b->pitch * sizeof(double), hipMemcpyDeviceToDevice, b->stream), "restore");
+    BCHK(b, hipMemcpyAsync(b->pos, b->kpos, pos_bytes(b), hipMemcpyDeviceToDevice, b->stream), "restore");
     BCHK(b, hipMemcpyAsync(b->rest, b->krest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "restore");
     BCHK(b, hipMemcpyAsync(b->crushed, b->kcrushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "restore");
     BCHK(b, hipMemcpyAsync(b->orig, b->korig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "restore");

@@ -29,7 +29,11 @@ def oracle_case(p0, soa0, nsub):
     return hist, pts
 
 
-@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 1e-9)],
+# Positions: FP64 storage keeps them to 1e-12 m.  FP32 storage holds them as
+# 32-bit fixed point over the grid width (quantum gx h / 2^32, ~3e-10 m on
+# these grids); the rounding of each substep's store walks a coordinate by
+# some tens of quanta over 150 substeps, hence 2e-8 m (2e-6 of a cell).
+@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 2e-8)],
                          ids=["f64", "f32"])
 def test_batch_matches_oracle_per_case(gpu_available, precision, tol_f, tol_x):
     print("batch", precision, flush=True)
