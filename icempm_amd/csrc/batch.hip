@@ -250,7 +250,6 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             F01 = (double)__builtin_nontemporal_load(&R[pp * 3 + pt]);
             F10 = (double)__builtin_nontemporal_load(&R[pp * 4 + pt]);
             F11 = fe0 + (double)__builtin_nontemporal_load(&R[pp * 5 + pt]);
-            Jp = (double)__builtin_nontemporal_load(&R[pp * 10 + pt]);
             crushed = __builtin_nontemporal_load(&S.crushed[pt]);
         }
         int bx = 0, by = 0;
@@ -333,7 +332,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         // ---- return map (convergent: the learned map runs wavefront-wide) ----
         // The trial SVD only for crushed points (few: the wavefront mostly
         // skips it); p and q for the NACC test came from the invariants.
+        // Jp_inv matters only to the map, so only crushed points load it.
         const bool crushedOk = ok && crushed == 1;
+        if (crushedOk) {
+            Jp = (double)R[pp * 10 + pt];
+            Jp0 = Jp;
+        }
         if constexpr (RMAP == kMapExact) {
             if (crushedOk && dp_projects(k, iv.p, iv.q, Jp))
                 dp_return(k, compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa), F00, F01, F10, F11, Jp);
