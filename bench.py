@@ -49,12 +49,13 @@ BATCH_F32_BYTES_PER_POINT = 108
 BATCH_F64_BYTES_PER_POINT = 212
 # The bytes the fused passes actually have to move are fewer: the next G2P
 # recomputes v and Bp before any use, so a fused pass neither reads nor
-# writes them, and Jp and the crushed flag are written only when they
-# change.  FP64 single case: read pos 16 + Fe 32 + Jp 8 + crushed 1 + grain
-# 2, write pos 16 + Fe 32 = 107 B; batched FP32 storage (positions 2 x 32-bit
-# fixed point, no grain row): read 8 + 16 + 4 + 1, write 8 + 16 = 53 B.
-FUSED_MOVED_BYTES_PER_POINT = 107
-BATCH_F32_MOVED_BYTES_PER_POINT = 53
+# writes them; Jp_inv is read only by crushed points, and it and the crushed
+# flag are written only when they change.  FP64 single case: read pos 16 +
+# Fe 32 + crushed 1 + grain 2, write pos 16 + Fe 32 = 99 B; batched FP32
+# storage (positions 2 x 32-bit fixed point, no grain row): read 8 + 16 + 1,
+# write 8 + 16 = 49 B.
+FUSED_MOVED_BYTES_PER_POINT = 99
+BATCH_F32_MOVED_BYTES_PER_POINT = 49
 METRIC = "particle-updates/s on 1 GPU: batched+learned vs plain MPM at ≤1% force err"
 
 
