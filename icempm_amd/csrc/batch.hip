@@ -714,8 +714,8 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
         }
         bool accepted = false;
         if (need) {
-            const double s0 = exp(log(r.svd.s0) + pred[0]);
-            const double s1 = exp(log(r.svd.s1) + pred[1]);
+            const double s0 = r.svd.s0 * exp(pred[0]);   // log s0 + pred0, exponentiated
+            const double s1 = r.svd.s1 * exp(pred[1]);
             // the shear branch leaves Jp_inv unchanged
             const double Jpn = (br == kBrTension) ? Jp * exp(pred[2]) : Jp;
             if (yield_residual(k, r, Jp, br, qlim, s0, s1, Jpn) <= la.tol) {
