@@ -420,6 +420,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         if (ok) {
             using PT = typename Rows<F32>::P;
             const PT qx = pos_put<PT>(px, cp), qy = pos_put<PT>(py, cp);
+            if constexpr (F32) {
+                // fixed point clamps to [0, gx h): a position outside (or
+                // NaN) is an error here, as the FP64 rows would report it
+                const double ux = px * cp.posInvQ, uy = py * cp.posInvQ;
+                if (!(ux >= 0.0 && ux < 4294967296.0 && uy >= 0.0 && uy < 4294967296.0)) flag_error(k);
+            }
             __builtin_nontemporal_store(qx, &S.pos[pt]);
             __builtin_nontemporal_store(qy, &S.pos[pp + pt]);
             if (!defer) {
