@@ -39,6 +39,7 @@ struct CaseParams {
     double ics, its, iss, tanPhi, dpThr, x0, y0;
     double pmax, msq, betaPmax, onePlus2Beta, slope;   // NACC constants of the case's material
     double posQ, posInvQ;   // FP32 storage: position quantum (grid width / 2^32) and its inverse
+    double scm;             // dt V_p D^-1 / m_p: the stress term of the affine momentum per unit mass
     int gx, gy, gridTotal, nPts, ptOffset, gridOffset, pad0, pad1;
 };
 
@@ -518,21 +519,22 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 kirchhoff_from_inv(rt, t00, t01, t11);
             else
                 kirchhoff(F00, F01, F10, F11, cp.mu, cp.kappa, t00, t01, t11);
+            // Every point of a case has the same mass m: the window sums the
+            // contributions per unit mass, W (v + (Bp - m^-1 dt V D^-1 tau) d),
+            // and the flush multiplies by m once per node.
             const double m = cp.mass, h = cp.h;
-            const double sc = cp.dt * cp.vol * cp.Dinv;
-            const double A00 = m * B00 - sc * t00, A01 = m * B01 - sc * t01;
-            const double A10 = m * B10 - sc * t01, A11 = m * B11 - sc * t11;
-            const double mv0 = vx * m, mv1 = vy * m;
+            const double A00 = B00 - cp.scm * t00, A01 = B01 - cp.scm * t01;
+            const double A10 = B10 - cp.scm * t01, A11 = B11 - cp.scm * t11;
             const int lx0 = bx - ox, ly0 = by - oy;
             // unsigned: a stencil left of or below the origin never lands in the window
             if (bx + 2 < gx && by + 2 < gy && (unsigned)lx0 + 2 < (unsigned)kWX &&
                 (unsigned)ly0 + 2 < (unsigned)kWY) {
                 const int e0 = ly0 * kWS + lx0;
                 double xw0 = wx0, xw1 = wx1, xw2 = wx2;
-                double xa0 = mv0 + A00 * ((0 - fx0) * h), xa1 = mv0 + A00 * ((1 - fx0) * h),
-                       xa2 = mv0 + A00 * ((2 - fx0) * h);
-                double xb0 = mv1 + A10 * ((0 - fx0) * h), xb1 = mv1 + A10 * ((1 - fx0) * h),
-                       xb2 = mv1 + A10 * ((2 - fx0) * h);
+                double xa0 = vx + A00 * ((0 - fx0) * h), xa1 = vx + A00 * ((1 - fx0) * h),
+                       xa2 = vx + A00 * ((2 - fx0) * h);
+                double xb0 = vy + A10 * ((0 - fx0) * h), xb1 = vy + A10 * ((1 - fx0) * h),
+                       xb2 = vy + A10 * ((2 - fx0) * h);
                 double yw0 = wy0, yw1 = wy1, yw2 = wy2;
                 double ya0 = A01 * ((0 - fx1) * h), ya1 = A01 * ((1 - fx1) * h), ya2 = A01 * ((2 - fx1) * h);
                 double yb0 = A11 * ((0 - fx1) * h), yb1 = A11 * ((1 - fx1) * h), yb2 = A11 * ((2 - fx1) * h);
@@ -555,7 +557,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                     for (int b = 0; b < 3; ++b) {
                         const double W = xw[a] * yw[b];
                         const int e = e0 + xe[a] + ye[b];
-                        atomicAdd(&W0[e], W * m);
+                        atomicAdd(&W0[e], W);
                         atomicAdd(&W1[e], W * (xa[a] + ya[b]));
                         atomicAdd(&W2[e], W * (xb[a] + yb[b]));
                     }
@@ -566,9 +568,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                     for (int j = 0; j < 3; ++j) {
                         const double Wip = sel3(i, wx0, wx1, wx2) * sel3(j, wy0, wy1, wy2);
                         const double d0 = (i - fx0) * h, d1 = (j - fx1) * h;
-                        const double incV0 = Wip * (mv0 + (A00 * d0 + A01 * d1));
-                        const double incV1 = Wip * (mv1 + (A10 * d0 + A11 * d1));
-                        const double incM = Wip * m;
+                        const double incV0 = Wip * (vx + (A00 * d0 + A01 * d1));
+                        const double incV1 = Wip * (vy + (A10 * d0 + A11 * d1));
                         const int X = bx + i, Y = by + j;
                         if (X >= gx || Y >= gy) {
                             flag_error(k);
@@ -576,14 +577,14 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                         }
                         if ((unsigned)(X - ox) < (unsigned)kWX && (unsigned)(Y - oy) < (unsigned)kWY) {
                             const int e = (Y - oy) * kWS + (X - ox);
-                            atomicAdd(&W0[e], incM);
+                            atomicAdd(&W0[e], Wip);
                             atomicAdd(&W1[e], incV0);
                             atomicAdd(&W2[e], incV1);
                         } else {
                             const int node = X + Y * gx;
-                            unsafeAtomicAdd(&gO[node], incM);
-                            unsafeAtomicAdd(&gO[gpitch + node], incV0);
-                            unsafeAtomicAdd(&gO[2 * gpitch + node], incV1);
+                            unsafeAtomicAdd(&gO[node], Wip * m);
+                            unsafeAtomicAdd(&gO[gpitch + node], incV0 * m);
+                            unsafeAtomicAdd(&gO[2 * gpitch + node], incV1 * m);
                         }
                     }
             }
@@ -598,9 +599,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                     const int X = ox + lx, Y = oy + ly;
                     if (X < gx && Y < gy) {
                         const int node = X + Y * gx;
-                        unsafeAtomicAdd(&gO[node], a);
-                        unsafeAtomicAdd(&gO[gpitch + node], W1[e]);
-                        unsafeAtomicAdd(&gO[2 * gpitch + node], W2[e]);
+                        const double m = cp.mass;   // the window holds sums per unit mass
+                        unsafeAtomicAdd(&gO[node], a * m);
+                        unsafeAtomicAdd(&gO[gpitch + node], W1[e] * m);
+                        unsafeAtomicAdd(&gO[2 * gpitch + node], W2[e] * m);
                     }
                 }
             }
@@ -1321,6 +1323,7 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
         c.slope = -(2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin)) / -3e6;
     }
     c.gx = P.GridX; c.gy = P.GridY; c.gridTotal = P.GridTotal; c.nPts = P.nPts;
+    c.scm = c.dt * c.vol * c.Dinv / c.mass;
     c.posQ = (c.gx * c.h) / 4294967296.0;   // FP32 storage: positions in [0, gx h) on 32 bits
     c.posInvQ = 1.0 / c.posQ;
     const int n = P.nPts;
