@@ -374,7 +374,7 @@ def run_batch_learned(args) -> dict:
             "cases": len(cases),
             "points": n_total,
             "grid_nodes": nodes,
-            "storage": "f32 (positions f64)",
+            "storage": "f32 (positions 32-bit fixed point over each grid width)",
             "mlp": "bf16 weights/activations, f32 accumulation",
             "points_per_chunk": chunk,
             "resort_every": resort_every,
