@@ -108,3 +108,32 @@ def test_resort_chunk_sizes_and_checkpoint_match_oracle(gpu_available):
     b.run(nsub)                                   # the state was restored by the tuner
     check("after autotune")
     b.close()
+
+
+@pytest.mark.parametrize("precision", [BATCH_F64, BATCH_F32], ids=["f64", "f32"])
+def test_batch_state_round_trip(gpu_available, precision):
+    """A finalized batch holds every case's points in band order and, for FP32
+    storage, positions in fixed point: read back before any substep, each case
+    returns its ingested state in the original point order — exactly for FP64
+    storage; for FP32 to within half a position quantum (gx h / 2^33) and the
+    FP32 rounding of the other rows."""
+    print("batch round trip", precision, flush=True)
+    cases = batch_cases(4, 700, 5000, seed=17)
+    b = build_batch(cases, precision)
+    for c, case in enumerate(cases):
+        p, soa0 = b.initial[c]
+        n = case.n_points
+        dev = b.points(c)
+        assert np.array_equal(dev[0].view(np.uint8)[:n], soa0[0].view(np.uint8)[:n])   # crushed flags
+        if precision == BATCH_F64:
+            for r in range(3, 16):   # Jp_inv, position, velocity, Fe, Bp
+                assert np.array_equal(dev[r, :n], soa0[r, :n]), r
+        else:
+            q = p.GridX * p.cellsize / 2.0 ** 32
+            assert np.abs(dev[4:6, :n] - soa0[4:6, :n]).max() <= 0.5 * q * (1 + 1e-9)
+            for r in list(range(6, 16)) + [3]:
+                ref = soa0[r, :n].astype(np.float32).astype(np.float64)
+                if r in (8, 11):   # Fe00, Fe11 are stored as Fe - 1
+                    ref = ((soa0[r, :n] - 1.0).astype(np.float32).astype(np.float64)) + 1.0
+                assert np.array_equal(dev[r, :n], ref), r
+    b.close()
