@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
     double *__restrict__ gO = gridOut + cp.gridOffset;
     using V = typename Rows<F32>::V;
     V *__restrict__ R = S.rest;
-    constexpr double fe0 = F32 ? 1.0 : 0.0;   // stored Fe = Fe - fe0 * I
+    constexpr double fe0 = F32 ? 1.0 : 0.0;   // stored Fe = Fe - fe0 * I, Jp_inv - fe0
 
     double px = 0, py = 0, vx = 0, vy = 0;
     double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         // Jp_inv matters only to the map, so only crushed points load it.
         const bool crushedOk = ok && crushed == 1;
         if (crushedOk) {
-            Jp = (double)R[pp * 10 + pt];
+            Jp = fe0 + (double)R[pp * 10 + pt];
             Jp0 = Jp;
         }
         if constexpr (RMAP == kMapExact) {
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 R[pp * 8 + pt] = (V)B10;
                 R[pp * 9 + pt] = (V)B11;
             }
-            if (Jp != Jp0 && !defer) R[pp * 10 + pt] = (V)Jp;
+            if (Jp != Jp0 && !defer) R[pp * 10 + pt] = (V)(Jp - fe0);
             if (crushed != crushed0) S.crushed[pt] = crushed;
             if constexpr (F32) {
                 // position and Fe are what the next substep reads back: the
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
             R[pp * 3 + pt] = (V)F01;
             R[pp * 4 + pt] = (V)F10;
             R[pp * 5 + pt] = (V)(F11 - fe0);
-            if (Jp != e.Jp) R[pp * 10 + pt] = (V)Jp;
+            if (Jp != e.Jp) R[pp * 10 + pt] = (V)(Jp - fe0);
             if (gridOut) scatter_point<F32>(e, cp, k, F00, F01, F10, F11, gridOut, gpitch);
         }
     }
@@ -1411,7 +1411,7 @@ int icempm_batch_finalize(icempm_batch *b) {
             for (int r = 0; r < 11; ++r) {
                 double v = h.rest[(size_t)r * n + i];
                 if (b->precision == ICEMPM_BATCH_F32) {
-                    if (r == 2 || r == 5) v -= 1.0;    // Fe00, Fe11 stored as Fe - I
+                    if (r == 2 || r == 5 || r == 10) v -= 1.0;    // Fe00, Fe11, Jp_inv stored less 1
                     float f = (float)v;
                     std::memcpy(&hrest[((size_t)r * b->pitch + o + k) * vs], &f, vs);
                 } else {
@@ -1629,7 +1629,7 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
                 float f;
                 std::memcpy(&f, &hrest[((size_t)r * n + slot) * vs], vs);
                 v = f;
-                if (r == 2 || r == 5) v += 1.0;
+                if (r == 2 || r == 5 || r == 10) v += 1.0;
             } else {
                 std::memcpy(&v, &hrest[((size_t)r * n + slot) * vs], vs);
             }

@@ -199,7 +199,7 @@ int icempm_read_bins(icempm_ctx *ctx, double *out);
  * by icempm_read_points after LoadRawPoints); all cases share the time step;
  * the material of a case must be uniform (GrainVariability = 0).
  * Precision: ICEMPM_BATCH_F64 stores the point state in FP64;
- * ICEMPM_BATCH_F32 stores velocity, Fe - I, Bp and Jp_inv in FP32 and the
+ * ICEMPM_BATCH_F32 stores velocity, Fe - I, Bp and Jp_inv - 1 in FP32 and the
  * positions as 32-bit fixed point over the case's grid width (a quantum of
  * GridX * cellsize / 2^32); the grid stays FP64 and all arithmetic is FP64.  The per-substep total
  * indenter force of every case is recorded; per-bin forces are not. */

@@ -133,7 +133,7 @@ def test_batch_state_round_trip(gpu_available, precision):
             assert np.abs(dev[4:6, :n] - soa0[4:6, :n]).max() <= 0.5 * q * (1 + 1e-9)
             for r in list(range(6, 16)) + [3]:
                 ref = soa0[r, :n].astype(np.float32).astype(np.float64)
-                if r in (8, 11):   # Fe00, Fe11 are stored as Fe - 1
+                if r in (8, 11, 3):   # Fe00, Fe11 and Jp_inv are stored less 1
                     ref = ((soa0[r, :n] - 1.0).astype(np.float32).astype(np.float64)) + 1.0
                 assert np.array_equal(dev[r, :n], ref), r
     b.close()
