@@ -53,6 +53,9 @@ template <bool F32>
 struct Rows {
     using V = typename std::conditional<F32, float, double>::type;
     using P = typename std::conditional<F32, unsigned int, double>::type;
+    // grid velocity as the gathers read it: FP64 rows 1-2 of the grid, or
+    // (FP32 storage) one FP32 pair per node written by the grid update
+    using G = typename std::conditional<F32, float2, double>::type;
     P *pos;            // 2 rows
     V *rest;           // 11 rows: vel(2) Fe(4) Bp(4) Jp(1)
     unsigned char *crushed;
@@ -71,6 +74,17 @@ template <>
 __host__ __device__ inline unsigned int pos_put<unsigned int>(double x, const CaseParams &cp) {
     const double u = rint(x * cp.posInvQ);
     return (u >= 0.0 && u <= 4294967295.0) ? (unsigned int)u : (u < 0.0 ? 0u : 4294967295u);
+}
+
+// the velocity of a node (index within its case's grid) as the gathers read it
+static __device__ __forceinline__ void grid_v(const double *g, int node, size_t gpitch, double &v0, double &v1) {
+    v0 = g[gpitch + node];
+    v1 = g[2 * gpitch + node];
+}
+static __device__ __forceinline__ void grid_v(const float2 *g, int node, size_t, double &v0, double &v1) {
+    const float2 a = g[node];
+    v0 = a.x;
+    v1 = a.y;
 }
 
 static __device__ __forceinline__ void wsync() {
@@ -208,7 +222,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                                                          const CaseParams *__restrict__ cases,
                                                          typename Rows<F32>::P *pos,
                                                          typename Rows<F32>::V *rest, unsigned char *crushedRow,
-                                                         size_t pitch, const double *__restrict__ gridV,
+                                                         size_t pitch, const typename Rows<F32>::G *__restrict__ gridV,
                                                          double *__restrict__ gridOut, size_t gpitch,
                                                          int *__restrict__ err, LearnArgs la, MapQueue mq) {
     const Rows<F32> S{pos, rest, crushedRow, pitch};
@@ -227,7 +241,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
     const bool valid = pt < end;
     const size_t pp = S.pitch;
     const int gx = cp.gx, gy = cp.gy;
-    const double *__restrict__ gV = gridV + cp.gridOffset;
+    const auto *__restrict__ gV = gridV + cp.gridOffset;
     double *__restrict__ gO = gridOut + cp.gridOffset;
     using V = typename Rows<F32>::V;
     V *__restrict__ R = S.rest;
@@ -263,8 +277,14 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         double Jp0 = Jp;
         if (ok) {
             double n0[9], n1[9];
-            if (bx + 2 < gx && by + 2 < gy) {
-                const double *gvx = gV + gpitch + bx + by * gx;
+            if (F32 && bx + 2 < gx && by + 2 < gy) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        grid_v(gV, bx + i + (by + j) * gx, gpitch, n0[i * 3 + j], n1[i * 3 + j]);
+            } else if (bx + 2 < gx && by + 2 < gy) {
+                const double *gvx = reinterpret_cast<const double *>(gV) + gpitch + bx + by * gx;
                 const double *gvy = gvx + gpitch;
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
@@ -284,8 +304,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                         if (X >= gx || Y >= gy) {
                             flag_error(k);
                         } else {
-                            n0[i * 3 + j] = gV[gpitch + X + Y * gx];
-                            n1[i * 3 + j] = gV[2 * gpitch + X + Y * gx];
+                            grid_v(gV, X + Y * gx, gpitch, n0[i * 3 + j], n1[i * 3 + j]);
                         }
                     }
             }
@@ -753,11 +772,13 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
 // Grid update over all cases: block b handles nodes [nodeBlock[b].y, +256)
 // of case nodeBlock[b].x.  Same arithmetic as the single-case update; the
 // substep's total indenter force of each case goes to hist[case*2 + comp].
+// The velocities go to rows 1-2 of the grid, or, when vel32 is given (FP32
+// point storage), to vel32 as FP32 pairs, zero for the nodes without mass.
 __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restrict__ cases,
                                                      const int2 *__restrict__ nodeBlock,
                                                      double *__restrict__ grid, double *__restrict__ zeroGrid,
                                                      size_t gpitch, const double *__restrict__ tTab, int step,
-                                                     double *__restrict__ hist) {
+                                                     double *__restrict__ hist, float2 *__restrict__ vel32) {
     __shared__ double sf[2];
     __shared__ int any;
     const int2 nb = nodeBlock[blockIdx.x];
@@ -777,6 +798,7 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
             zeroGrid[2 * gpitch + idx] = 0;
         }
         const double mass = grid[idx];
+        if (vel32 && mass == 0) vel32[idx] = make_float2(0.f, 0.f);
         if (mass != 0) {
             const double dt = cp.dt, h = cp.h, V = cp.indV;
             const double t = tTab[step];
@@ -819,8 +841,12 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
                 v0 = 0;
             else if (ix >= cp.gx - 3 && v0 > 0)
                 v0 = 0;
-            grid[gpitch + idx] = v0;
-            grid[2 * gpitch + idx] = v1;
+            if (vel32) {
+                vel32[idx] = make_float2((float)v0, (float)v1);
+            } else {
+                grid[gpitch + idx] = v0;
+                grid[2 * gpitch + idx] = v1;
+            }
         }
     }
     __syncthreads();
@@ -898,6 +924,7 @@ struct icempm_batch {
     size_t pitch = 0;
     int nPts = 0;
     double *gridA = nullptr, *gridB = nullptr;
+    float2 *vel32 = nullptr;   // FP32 storage: the grid velocities the gathers read
     size_t gpitch = 0;
     long nNodes = 0;
     CaseParams *cases = nullptr;
@@ -990,12 +1017,13 @@ void dfree(T *&p) {
 constexpr int kMapBlocks = 256;
 
 template <bool F32, bool G2P, bool P2G, int RMAP>
-void launch_one(icempm_batch *b, unsigned nb, const double *gV, double *gO, const LearnArgs &la,
+void launch_one(icempm_batch *b, unsigned nb, const void *gV, double *gO, const LearnArgs &la,
                 const MapQueue &mq) {
     Rows<F32> S{static_cast<typename Rows<F32>::P *>(b->pos), reinterpret_cast<typename Rows<F32>::V *>(b->rest),
                 b->crushed, b->pitch};
     hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream,
-                       b->chunks, b->nChunks, b->cases, S.pos, S.rest, S.crushed, S.pitch, gV, gO, b->gpitch,
+                       b->chunks, b->nChunks, b->cases, S.pos, S.rest, S.crushed, S.pitch,
+                       static_cast<const typename Rows<F32>::G *>(gV), gO, b->gpitch,
                        b->err, la, mq);
     if (b->evAfter) (void)hipEventRecord(b->evAfter, b->stream);
     if constexpr (G2P && RMAP == kMapLearned) {
@@ -1007,7 +1035,7 @@ void launch_one(icempm_batch *b, unsigned nb, const double *gV, double *gO, cons
 }
 
 template <bool F32, int RMAP>
-void launch_map(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const double *gV, double *gO,
+void launch_map(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const void *gV, double *gO,
                 const LearnArgs &la, const MapQueue &mq) {
     if (g2p && p2g)
         launch_one<F32, true, true, RMAP>(b, nb, gV, gO, la, mq);
@@ -1018,7 +1046,7 @@ void launch_map(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const double *
 }
 
 template <bool F32>
-void launch_prec(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const double *gV, double *gO,
+void launch_prec(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const void *gV, double *gO,
                  const LearnArgs &la, const MapQueue &mq) {
     if (b->rmap == kMapHarvest)
         launch_map<F32, kMapHarvest>(b, nb, g2p, p2g, gV, gO, la, mq);
@@ -1030,7 +1058,8 @@ void launch_prec(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const double 
 
 // One particle pass (and, with the learned map, the map kernel after it);
 // parity selects the queue counter the pass appends to.
-hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const double *gV, double *gO, int parity) {
+// gV: the grid whose rows 1-2 hold the velocities (FP64 storage) or vel32
+hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const void *gV, double *gO, int parity) {
     const unsigned nb = (unsigned)(((b->nChunks + 7) / 8) * 8);
     if (b->nChunks <= 0) return hipSuccess;
     LearnArgs la{};
@@ -1218,8 +1247,9 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
         const bool last = (i == n - 1);
         hipLaunchKernelGGL(k_batch_update, dim3(b->nNodeBlocks), dim3(256), 0, b->stream, b->cases,
                            b->nodeBlock, A, last ? nullptr : B, b->gpitch, tTab, i,
-                           hist + 2 * nc * (size_t)i);
+                           hist + 2 * nc * (size_t)i, b->vel32);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        const void *gV = b->vel32 ? static_cast<const void *>(b->vel32) : static_cast<const void *>(A);
         if (!last) {
             const bool timeIt = b->timingStride > 0 && i % b->timingStride == 0;
             hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -1237,12 +1267,12 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
                 if ((e = hipEventRecord(t0, b->stream)) != hipSuccess) return e;
             }
             b->evAfter = timeIt ? t1 : nullptr;   // recorded right after the particle kernel
-            e = launch_pass(b, true, true, A, B, i);
+            e = launch_pass(b, true, true, gV, B, i);
             b->evAfter = nullptr;
             if (e != hipSuccess) return e;
             std::swap(A, B);
         } else {
-            if ((e = launch_pass(b, true, false, A, nullptr, i)) != hipSuccess) return e;
+            if ((e = launch_pass(b, true, false, gV, nullptr, i)) != hipSuccess) return e;
         }
     }
     return hipSuccess;
@@ -1273,7 +1303,7 @@ void icempm_batch_destroy(icempm_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    dfree(b->pos); dfree(b->rest); dfree(b->crushed); dfree(b->gridA); dfree(b->gridB);
+    dfree(b->pos); dfree(b->rest); dfree(b->crushed); dfree(b->gridA); dfree(b->gridB); dfree(b->vel32);
     dfree(b->cases); dfree(b->chunks); dfree(b->nodeBlock);
     dfree(b->err); dfree(b->tTab); dfree(b->hist);
     dfree(b->mlpW); dfree(b->mlpP); dfree(b->hx); dfree(b->ht); dfree(b->hcount); dfree(b->stats);
@@ -1389,6 +1419,8 @@ int icempm_batch_finalize(icempm_batch *b) {
     BCHK(b, hipMalloc(&b->orig, b->pitch * sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->gridA, 3 * b->gpitch * sizeof(double)), "batch finalize");
     BCHK(b, hipMalloc(&b->gridB, 3 * b->gpitch * sizeof(double)), "batch finalize");
+    if (b->precision == ICEMPM_BATCH_F32)
+        BCHK(b, hipMalloc(&b->vel32, b->gpitch * sizeof(float2)), "batch finalize");
     BCHK(b, hipMalloc(&b->cases, b->hc.size() * sizeof(CaseParams)), "batch finalize");
     BCHK(b, hipMalloc(&b->nodeBlock, nblk.size() * sizeof(int2)), "batch finalize");
     BCHK(b, hipMalloc(&b->err, sizeof(int)), "batch finalize");

@@ -201,7 +201,8 @@ int icempm_read_bins(icempm_ctx *ctx, double *out);
  * Precision: ICEMPM_BATCH_F64 stores the point state in FP64;
  * ICEMPM_BATCH_F32 stores velocity, Fe - I, Bp and Jp_inv - 1 in FP32 and the
  * positions as 32-bit fixed point over the case's grid width (a quantum of
- * GridX * cellsize / 2^32); the grid stays FP64 and all arithmetic is FP64.  The per-substep total
+ * GridX * cellsize / 2^32); the grid sums stay FP64, the grid velocities
+ * reach the gathers as FP32 pairs, and all arithmetic is FP64.  The per-substep total
  * indenter force of every case is recorded; per-bin forces are not. */
 typedef struct icempm_batch icempm_batch;
 enum { ICEMPM_BATCH_F64 = 0, ICEMPM_BATCH_F32 = 1 };
