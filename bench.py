@@ -56,6 +56,9 @@ BATCH_F64_BYTES_PER_POINT = 212
 # write 8 + 16 = 49 B.
 FUSED_MOVED_BYTES_PER_POINT = 99
 BATCH_F32_MOVED_BYTES_PER_POINT = 49
+# With FP32 point storage the batched gathers read each node's velocity as
+# an FP32 pair: 24 B written by the P2G + 8 B read per node.
+BATCH_F32_MOVED_BYTES_PER_NODE = 32
 METRIC = "particle-updates/s on 1 GPU: batched+learned vs plain MPM at ≤1% force err"
 
 
@@ -307,7 +310,7 @@ def run_batch_learned(args) -> dict:
     b.close()
     k_avg_s = (ktot / max(klaunch, 1)) / 1e3
     k_bytes = n_total * BATCH_F32_BYTES_PER_POINT + nodes * FUSED_BYTES_PER_NODE
-    k_moved = n_total * BATCH_F32_MOVED_BYTES_PER_POINT + nodes * FUSED_BYTES_PER_NODE
+    k_moved = n_total * BATCH_F32_MOVED_BYTES_PER_POINT + nodes * BATCH_F32_MOVED_BYTES_PER_NODE
     achieved = k_bytes / k_avg_s / 1e9 if klaunch else None
     log(f"bench: fused pass {k_avg_s * 1e6:.1f} us ({klaunch} timed launches), "
         f"{achieved:.0f} GB/s algorithmic")
