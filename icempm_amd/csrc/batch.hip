@@ -277,22 +277,25 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         double Jp0 = Jp;
         if (ok) {
             double n0[9], n1[9];
-            if (F32 && bx + 2 < gx && by + 2 < gy) {
+            if (bx + 2 < gx && by + 2 < gy) {
+                if constexpr (F32) {
+                    // one FP32 pair per node, a row of three at a time
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
+                    for (int j = 0; j < 3; ++j)
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+                            grid_v(gV, bx + i + (by + j) * gx, gpitch, n0[i * 3 + j], n1[i * 3 + j]);
+                } else {
+                    const double *gvx = gV + gpitch + bx + by * gx;
+                    const double *gvy = gvx + gpitch;
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
-                        grid_v(gV, bx + i + (by + j) * gx, gpitch, n0[i * 3 + j], n1[i * 3 + j]);
-            } else if (bx + 2 < gx && by + 2 < gy) {
-                const double *gvx = reinterpret_cast<const double *>(gV) + gpitch + bx + by * gx;
-                const double *gvy = gvx + gpitch;
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        n0[i * 3 + j] = gvx[i + j * gx];
-                        n1[i * 3 + j] = gvy[i + j * gx];
-                    }
+                        for (int j = 0; j < 3; ++j) {
+                            n0[i * 3 + j] = gvx[i + j * gx];
+                            n1[i * 3 + j] = gvy[i + j * gx];
+                        }
+                }
             } else {
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
