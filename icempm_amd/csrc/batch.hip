@@ -777,11 +777,19 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
 // substep's total indenter force of each case goes to hist[case*2 + comp].
 // The velocities go to rows 1-2 of the grid, or, when vel32 is given (FP32
 // point storage), to vel32 as FP32 pairs, zero for the nodes without mass.
+// Per-block flags skip the stores that would write zeros over zeros: a
+// block of zeroGrid is cleared only if it took mass in its last P2G
+// (zeroHasMass, recorded when that grid was last updated), and a block of
+// vel32 is rewritten at its massless nodes only if it last held a velocity
+// (velNonzero).  The host sets every flag at the start of a run.
 __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restrict__ cases,
                                                      const int2 *__restrict__ nodeBlock,
                                                      double *__restrict__ grid, double *__restrict__ zeroGrid,
                                                      size_t gpitch, const double *__restrict__ tTab, int step,
-                                                     double *__restrict__ hist, float2 *__restrict__ vel32) {
+                                                     double *__restrict__ hist, float2 *__restrict__ vel32,
+                                                     int *__restrict__ gridHasMass,
+                                                     const int *__restrict__ zeroHasMass,
+                                                     int *__restrict__ velNonzero) {
     __shared__ double sf[2];
     __shared__ int any;
     const int2 nb = nodeBlock[blockIdx.x];
@@ -791,17 +799,26 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
         sf[1] = 0;
         any = 0;
     }
-    __syncthreads();
+    // a block of zeroGrid that took no mass in its last P2G is still zero,
+    // and a block of vel32 last written with no mass holds zeros
+    const bool zeroIt = zeroGrid && zeroHasMass[blockIdx.x] != 0;
+    const bool velDirty = vel32 && velNonzero[blockIdx.x] != 0;
     const int local = nb.y + (int)threadIdx.x;
-    if (local < cp.gridTotal) {
-        const size_t idx = (size_t)cp.gridOffset + local;
-        if (zeroGrid) {
+    const bool node = local < cp.gridTotal;
+    const size_t idx = (size_t)cp.gridOffset + (node ? local : 0);
+    const double mass = node ? grid[idx] : 0.0;
+    const bool blockMass = __syncthreads_or(mass != 0) != 0;   // also orders sf/any
+    if (threadIdx.x == 0) {
+        gridHasMass[blockIdx.x] = blockMass ? 1 : 0;
+        if (vel32) velNonzero[blockIdx.x] = blockMass ? 1 : 0;
+    }
+    if (node) {
+        if (zeroIt) {
             zeroGrid[idx] = 0;
             zeroGrid[gpitch + idx] = 0;
             zeroGrid[2 * gpitch + idx] = 0;
         }
-        const double mass = grid[idx];
-        if (vel32 && mass == 0) vel32[idx] = make_float2(0.f, 0.f);
+        if (velDirty && mass == 0) vel32[idx] = make_float2(0.f, 0.f);
         if (mass != 0) {
             const double dt = cp.dt, h = cp.h, V = cp.indV;
             const double t = tTab[step];
@@ -928,6 +945,7 @@ struct icempm_batch {
     int nPts = 0;
     double *gridA = nullptr, *gridB = nullptr;
     float2 *vel32 = nullptr;   // FP32 storage: the grid velocities the gathers read
+    int *blkFlags = nullptr;   // per node block: grid A took mass, grid B took mass, vel32 nonzero
     size_t gpitch = 0;
     long nNodes = 0;
     CaseParams *cases = nullptr;
@@ -1245,12 +1263,16 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
         return e;
     if ((e = launch_pass(b, false, true, b->gridA, b->gridA, 0)) != hipSuccess) return e;
     double *A = b->gridA, *B = b->gridB;
+    const size_t nblk = (size_t)b->nNodeBlocks;
+    int *fA = b->blkFlags, *fB = b->blkFlags + nblk, *fV = b->blkFlags + 2 * nblk;
+    // the grids' and vel32's contents are unknown at the start of a run
+    if ((e = launch_fill_i32(b->blkFlags, 3 * nblk, 1, b->stream)) != hipSuccess) return e;
     const size_t nc = b->hc.size();
     for (int i = 0; i < n; ++i) {
         const bool last = (i == n - 1);
         hipLaunchKernelGGL(k_batch_update, dim3(b->nNodeBlocks), dim3(256), 0, b->stream, b->cases,
                            b->nodeBlock, A, last ? nullptr : B, b->gpitch, tTab, i,
-                           hist + 2 * nc * (size_t)i, b->vel32);
+                           hist + 2 * nc * (size_t)i, b->vel32, fA, fB, fV);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         const void *gV = b->vel32 ? static_cast<const void *>(b->vel32) : static_cast<const void *>(A);
         if (!last) {
@@ -1274,6 +1296,7 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
             b->evAfter = nullptr;
             if (e != hipSuccess) return e;
             std::swap(A, B);
+            std::swap(fA, fB);
         } else {
             if ((e = launch_pass(b, true, false, gV, nullptr, i)) != hipSuccess) return e;
         }
@@ -1307,6 +1330,7 @@ void icempm_batch_destroy(icempm_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     dfree(b->pos); dfree(b->rest); dfree(b->crushed); dfree(b->gridA); dfree(b->gridB); dfree(b->vel32);
+    dfree(b->blkFlags);
     dfree(b->cases); dfree(b->chunks); dfree(b->nodeBlock);
     dfree(b->err); dfree(b->tTab); dfree(b->hist);
     dfree(b->mlpW); dfree(b->mlpP); dfree(b->hx); dfree(b->ht); dfree(b->hcount); dfree(b->stats);
@@ -1426,6 +1450,7 @@ int icempm_batch_finalize(icempm_batch *b) {
         BCHK(b, hipMalloc(&b->vel32, b->gpitch * sizeof(float2)), "batch finalize");
     BCHK(b, hipMalloc(&b->cases, b->hc.size() * sizeof(CaseParams)), "batch finalize");
     BCHK(b, hipMalloc(&b->nodeBlock, nblk.size() * sizeof(int2)), "batch finalize");
+    BCHK(b, hipMalloc(&b->blkFlags, 3 * nblk.size() * sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->err, sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->hcount, sizeof(unsigned long long)), "batch finalize");
     BCHK(b, hipMalloc(&b->stats, 3 * sizeof(unsigned long long)), "batch finalize");
