@@ -105,6 +105,20 @@ def run_cm_batch(tmp_path, cases, f32):
     return refs, N, hists
 
 
+def oracle_histories(refs, N):
+    """The oracle's ingest and N substeps of each case alone."""
+    out = []
+    for p, xy, g, vol in refs:
+        pts = oracle.ingest(p, xy, g, vol, (xy.shape[0] + 63) // 64 * 64)
+        p.nGridPitch = (p.GridTotal + 63) // 64 * 64
+        grid = oracle.new_grid(p)
+        bins = np.zeros(2 * p.n_indenter_subdivisions)
+        href, err = oracle.run(p, pts, grid, bins, N)
+        assert err == 0 and np.abs(href).sum() > 0
+        out.append(href)
+    return out
+
+
 def test_cm_batch_manifest_matches_oracle(gpu_available, tmp_path):
     """cm --batch: configurations listed in a manifest run as one batch (FP64
     storage); each case's per-substep indenter force over one output cycle
@@ -112,28 +126,27 @@ def test_cm_batch_manifest_matches_oracle(gpu_available, tmp_path):
     from icempm_amd.batch import batch_cases
     print("cli: cm --batch, 3 cases, 1 cycle f64", flush=True)
     refs, N, hists = run_cm_batch(tmp_path, batch_cases(3, 1500, 4000, seed=21), False)
-    for k, ((p, xy, g, vol), h) in enumerate(zip(refs, hists)):
-        pts = oracle.ingest(p, xy, g, vol, (xy.shape[0] + 63) // 64 * 64)
-        p.nGridPitch = (p.GridTotal + 63) // 64 * 64
-        grid = oracle.new_grid(p)
-        bins = np.zeros(2 * p.n_indenter_subdivisions)
-        href, err = oracle.run(p, pts, grid, bins, N)
-        assert err == 0
+    for k, (href, h) in enumerate(zip(oracle_histories(refs, N), hists)):
         e = np.linalg.norm(h - href) / np.linalg.norm(href)
         print(f"case {k}: force rel L2 {e:.2e}", flush=True)
-        assert np.abs(href).sum() > 0 and e <= 1e-6, (k, e)
+        assert e <= 1e-6, (k, e)
 
 
-def test_cm_batch_manifest_f32_matches_library(gpu_available, tmp_path):
-    """cm --batch with FP32 storage gives the library's FP32 batch run of the
-    same cases (icempm_amd.batch) to the run-to-run reproducibility of the
-    batched pass (hardware atomics; 1e-4 over the 2,500-substep cycle).  The
-    FP32 storage error itself is held to the oracle in test_gpu_batch."""
+def test_cm_batch_manifest_f32(gpu_available, tmp_path):
+    """cm --batch with FP32 storage: each case's force over the 2,500-substep
+    cycle stays within the FP32 bar of the oracle (1e-3 relative L2; measured
+    ~4e-6), and equals the library's FP32 batch run of the same cases
+    (icempm_amd.batch) to the run-to-run reproducibility of the batched pass
+    (hardware atomics; 1e-4)."""
     from icempm_amd._lib import BATCH_F32
     from icempm_amd.batch import batch_cases, build_batch
     print("cli: cm --batch, 3 cases, 1 cycle f32", flush=True)
     cases = batch_cases(3, 1500, 4000, seed=21)
-    _, N, hists = run_cm_batch(tmp_path, cases, True)
+    refs, N, hists = run_cm_batch(tmp_path, cases, True)
+    for k, (href, h) in enumerate(zip(oracle_histories(refs, N), hists)):
+        e = np.linalg.norm(h - href) / np.linalg.norm(href)
+        print(f"case {k}: force rel L2 vs oracle {e:.2e}", flush=True)
+        assert e <= 1e-3, (k, e)
     b = build_batch(cases, BATCH_F32)
     b.run(N)
     assert b.error_flag() == 0
