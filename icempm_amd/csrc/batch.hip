@@ -81,7 +81,7 @@ static __device__ __forceinline__ void grid_v(const double *g, int node, size_t 
     v0 = g[gpitch + node];
     v1 = g[2 * gpitch + node];
 }
-static __device__ __forceinline__ void grid_v(const float2 *g, int node, size_t, double &v0, double &v1) {
+static __device__ __forceinline__ void grid_v(const float2 *g, int node, size_t, float &v0, float &v1) {
     const float2 a = g[node];
     v0 = a.x;
     v1 = a.y;
@@ -276,7 +276,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         unsigned char crushed0 = crushed;
         double Jp0 = Jp;
         if (ok) {
-            double n0[9], n1[9];
+            // the gathered velocities stay in their stored type (FP32 pairs
+            // or FP64) and widen at each use: fewer live registers
+            using GT = typename std::conditional<F32, float, double>::type;
+            GT n0[9], n1[9];
             if (bx + 2 < gx && by + 2 < gy) {
                 if constexpr (F32) {
                     // one FP32 pair per node, a row of three at a time
@@ -317,10 +320,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             const double wxs[3] = {wx0, wx1, wx2};
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const double s0 = wy0 * n0[i * 3] + wy1 * n0[i * 3 + 1] + wy2 * n0[i * 3 + 2];
-                const double s1 = wy0 * n1[i * 3] + wy1 * n1[i * 3 + 1] + wy2 * n1[i * 3 + 2];
-                const double t0 = wyd0 * n0[i * 3] + wyd1 * n0[i * 3 + 1] + wyd2 * n0[i * 3 + 2];
-                const double t1 = wyd0 * n1[i * 3] + wyd1 * n1[i * 3 + 1] + wyd2 * n1[i * 3 + 2];
+                const double a0 = n0[i * 3], a1 = n0[i * 3 + 1], a2 = n0[i * 3 + 2];
+                const double c0 = n1[i * 3], c1 = n1[i * 3 + 1], c2 = n1[i * 3 + 2];
+                const double s0 = wy0 * a0 + wy1 * a1 + wy2 * a2;
+                const double s1 = wy0 * c0 + wy1 * c1 + wy2 * c2;
+                const double t0 = wyd0 * a0 + wyd1 * a1 + wyd2 * a2;
+                const double t1 = wyd0 * c0 + wyd1 * c1 + wyd2 * c2;
                 const double wxd = wxs[i] * (i - fx0);
                 v0 += wxs[i] * s0;
                 v1 += wxs[i] * s1;
