@@ -3,18 +3,20 @@
 // per substep cover every case, and the substep loop is captured once in a
 // HIP graph and replayed.
 //
-// Storage (per batch, cases contiguous, each case cell-sorted):
-//   positions      2 rows of double
-//   F64 variant    vel(2) Fe(4) Bp(4) Jp(1): 11 rows of double
-//   F32 variant    vel(2) Fe-I(4) Bp(4) Jp(1): 11 rows of float.  Fe is stored
-//                  as its deviation from the identity so that float keeps the
-//                  strain (|Fe-I| ~ 1e-5) to ~1e-12; positions stay double
-//                  because a substep moves a point by less than a float ulp
-//                  of its coordinate.
+// Storage (per batch, cases contiguous, each case in band order):
+//   F64 variant    positions: 2 rows of double; vel(2) Fe(4) Bp(4) Jp_inv(1):
+//                  11 rows of double
+//   F32 variant    positions: 2 rows of 32-bit fixed point over the case's
+//                  grid width; vel(2) Fe-I(4) Bp(4) Jp_inv-1(1): 11 rows of
+//                  float.  Fe and Jp_inv are stored as their deviations from 1
+//                  so that float keeps them to ~1e-12 absolute.
 //   crushed        1 byte per point
-//   grid           3 rows of double per node, cases concatenated
+//   grids          two grids of 3 rows of double per node (mass, momentum;
+//                  velocity after the update), cases concatenated, swapped
+//                  every substep; the F32 variant also keeps one FP32
+//                  velocity pair per node for the gathers
 // All arithmetic is FP64 in both variants (the return map included); the F32
-// variant only stores less: 93 B per point and substep instead of 164 B.
+// variant only stores less: a fused pass moves 49 B per point instead of 97 B.
 // Material is uniform per case (one set of NACC constants per case).
 #include <algorithm>
 #include <chrono>
