@@ -8,7 +8,7 @@ import pytest
 import oracle
 from icempm_amd import SimParams
 from icempm_amd._lib import BATCH_F32, BATCH_F64
-from icempm_amd.batch import batch_cases, build_batch
+from icempm_amd.batch import batch_cases, build_batch, ingest_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -136,4 +136,33 @@ def test_batch_state_round_trip(gpu_available, precision):
                 if r in (8, 11, 3):   # Fe00, Fe11 and Jp_inv are stored less 1
                     ref = ((soa0[r, :n] - 1.0).astype(np.float32).astype(np.float64)) + 1.0
                 assert np.array_equal(dev[r, :n], ref), r
+    b.close()
+
+
+@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 2e-8)],
+                         ids=["f64", "f32"])
+def test_mass_entering_empty_grid_blocks(gpu_available, precision, tol_f, tol_x):
+    """Every point starts with an upward velocity of 50 m/s, so over 300
+    substeps the block rises by ~2.5 cells into grid rows that held no mass
+    before: the grid update, which skips clearing blocks that took no mass,
+    must still give the oracle's run."""
+    print("batch: mass entering empty blocks", precision, flush=True)
+    cases = batch_cases(2, 2000, 4000, seed=31)
+    states = []
+    for p, soa in ingest_cases(cases):
+        soa = soa.copy()
+        soa[7, :] = 50.0       # velocity y of every point (rows past the points are padding)
+        states.append((p, soa))
+    nsub = 300
+    b = build_batch(cases, precision, states)
+    b.run(nsub)
+    assert b.error_flag() == 0
+    for c, case in enumerate(cases):
+        n = case.n_points
+        href, pref = oracle_case(*states[c], nsub)
+        dev = b.points(c)
+        rise = pref[5, :n].mean() - states[c][1][5, :n].mean()
+        assert rise > 2 * states[c][0].cellsize, rise
+        assert rel_l2(b.force_history(c, nsub), href) <= tol_f
+        assert np.abs(dev[4:6, :n] - pref[4:6, :n]).max() <= tol_x
     b.close()
