@@ -399,6 +399,9 @@ def run_batch_learned(args) -> dict:
             "timed_launches": klaunch,
             "copy_kernel_gbs": copy_gbs,
         },
+        # the whole substep (grid update and map kernel included) at the
+        # per-unit figure for FP32 state: 108 B per point + 40 B per node
+        "substep_effective_gbs": k_bytes * steps / (ms / 1e3) / 1e9,
         "comparison": {
             "plain_single_case_f64_pups": plain_pups,
             "plain_steps_per_case": args.plain_steps,
