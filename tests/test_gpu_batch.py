@@ -1,13 +1,14 @@
 """Batches of independent cases (one launch per kernel per substep for all
 cases, HIP graph replay) against the CPU oracle run case by case."""
 import ctypes
+import dataclasses
 
 import numpy as np
 import pytest
 
 import oracle
-from icempm_amd import SimParams
-from icempm_amd._lib import BATCH_F32, BATCH_F64
+from icempm_amd import IndentationCase, SimParams
+from icempm_amd._lib import BATCH_F32, BATCH_F64, IcempmError
 from icempm_amd.batch import batch_cases, build_batch, ingest_cases
 
 pytestmark = pytest.mark.gpu
@@ -166,3 +167,32 @@ def test_mass_entering_empty_grid_blocks(gpu_available, precision, tol_f, tol_x)
         assert rel_l2(b.force_history(c, nsub), href) <= tol_f
         assert np.abs(dev[4:6, :n] - pref[4:6, :n]).max() <= tol_x
     b.close()
+
+
+def test_grid_at_the_width_limit(gpu_available):
+    """A case 32,767 cells wide, the widest grid a batch takes (cell indices
+    travel as 16-bit halves in the particle pass; 2.1M points), follows the
+    oracle for 20 substeps with FP64 and FP32 storage; a grid one cell wider
+    is refused when the case is added."""
+    case = IndentationCase(name="widest", ncx=32767 - 56, ncy=16, per_side=2, h=0.012, grid_x=32767,
+                           grid_y=16 + 53, dt=2e-6, steps=20, ind_velocity=1.0)
+    print("batch: widest grid", case.n_points, "points", flush=True)
+    nsub = 20
+    states = ingest_cases([case])
+    href, pref = oracle_case(*states[0], nsub)
+    n = case.n_points
+    q = case.grid_x * case.h / 2.0 ** 32      # the FP32 position quantum on this grid, 9e-8 m
+    for precision, tol_f, tol_x in ((BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 40 * q)):
+        b = build_batch([case], precision, states)
+        b.run(nsub)
+        assert b.error_flag() == 0
+        e = rel_l2(b.force_history(0, nsub), href)
+        dx = np.abs(b.points(0)[4:6, :n] - pref[4:6, :n]).max()
+        print(f"precision {precision}: force rel L2 {e:.2e}, positions {dx:.2e} m", flush=True)
+        assert np.abs(href).sum() > 0 and e <= tol_f
+        assert dx <= tol_x
+        b.close()
+    small = batch_cases(1, 2000, 2000, seed=41)[0]
+    too_wide = dataclasses.replace(small, grid_x=32768)
+    with pytest.raises(IcempmError, match="32767"):
+        build_batch([too_wide], BATCH_F64)
