@@ -8,8 +8,8 @@ import pytest
 
 import oracle
 from icempm_amd import IndentationCase, SimParams
-from icempm_amd._lib import BATCH_F32, BATCH_F64, IcempmError
-from icempm_amd.batch import batch_cases, build_batch, ingest_cases
+from icempm_amd._lib import BATCH_F32, BATCH_F64, ICEMPM_E_ARG, ICEMPM_E_STATE, IcempmError
+from icempm_amd.batch import Batch, batch_cases, build_batch, ingest_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -196,3 +196,45 @@ def test_grid_at_the_width_limit(gpu_available):
     too_wide = dataclasses.replace(small, grid_x=32768)
     with pytest.raises(IcempmError, match="32767"):
         build_batch([too_wide], BATCH_F64)
+
+
+def test_batch_call_order_and_argument_errors(gpu_available):
+    """The batch calls refuse work out of order and bad arguments with a
+    status code and a message naming the problem, and the batch stays
+    usable afterwards."""
+    print("batch: call order and argument errors", flush=True)
+    cases = batch_cases(2, 1500, 3000, seed=51)
+    states = ingest_cases(cases)
+
+    def refused(code, fragment, call, *args, **kw):
+        with pytest.raises(IcempmError) as ei:
+            call(*args, **kw)
+        assert ei.value.code == code and fragment in str(ei.value), str(ei.value)
+
+    b = Batch(BATCH_F64)
+    refused(ICEMPM_E_ARG, "", b.finalize)                        # no case yet
+    refused(ICEMPM_E_STATE, "not finalized", b.run, 10)
+    refused(ICEMPM_E_STATE, "not finalized", b.checkpoint)
+    p, soa = states[0]
+    other_dt = states[1][0].copy()
+    other_dt.InitialTimeStep = p.InitialTimeStep * 2
+    b.add_state(cases[0], p, soa)
+    refused(ICEMPM_E_ARG, "time step", b.add_state, cases[1], other_dt, states[1][1])
+    grains = states[1][0].copy()
+    grains.GrainVariability = 0.5
+    refused(ICEMPM_E_ARG, "uniform material", b.add_state, cases[1], grains, states[1][1])
+    b.add_state(cases[1], *states[1])
+    b.finalize()
+    refused(ICEMPM_E_STATE, "already finalized", b.finalize)
+    refused(ICEMPM_E_STATE, "already finalized", b.add_state, cases[0], p, soa)
+    refused(ICEMPM_E_ARG, "64, 128 or 256", b.set_tuning, chunk=100)
+    refused(ICEMPM_E_ARG, ">= 0", b.set_tuning, resort_every=-1)
+    refused(ICEMPM_E_ARG, "positive capacity", b.set_harvest, 0)
+    refused(ICEMPM_E_ARG, "", b.force_history, 2, 1)            # no case 2
+    b.run(30)
+    refused(ICEMPM_E_ARG, "", b.force_history, 0, 31)           # longer than the last run
+    # still usable: the run equals the oracle
+    href, pref = oracle_case(*states[0], 30)
+    assert rel_l2(b.force_history(0, 30), href) <= 1e-6
+    assert b.error_flag() == 0
+    b.close()
