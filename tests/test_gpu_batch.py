@@ -238,3 +238,28 @@ def test_batch_call_order_and_argument_errors(gpu_available):
     assert rel_l2(b.force_history(0, 30), href) <= 1e-6
     assert b.error_flag() == 0
     b.close()
+
+
+@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 2e-8)],
+                         ids=["f64", "f32"])
+def test_tiny_and_ragged_cases(gpu_available, precision, tol_f, tol_x):
+    """Cases of 4, 60, 272 and 264 points in one batch (one cell; a case
+    smaller than a wavefront; cases that end one cell row or a few points
+    into a chunk) each follow the oracle."""
+    print("batch: tiny and ragged cases", precision, flush=True)
+    shapes = [(1, 1), (3, 5), (17, 4), (33, 2)]
+    cases = [IndentationCase(name=f"ragged{i}", ncx=nx, ncy=ny, per_side=2, h=0.012, grid_x=nx + 56,
+                             grid_y=ny + 53, dt=2e-6, steps=50, ind_diameter=0.05, ind_velocity=0.5,
+                             ind_depth=0.01, seed=0x1CE5EED + i)
+             for i, (nx, ny) in enumerate(shapes)]
+    assert [c.n_points for c in cases] == [4, 60, 272, 264]
+    nsub = 50
+    b = build_batch(cases, precision)
+    b.run(nsub)
+    assert b.error_flag() == 0
+    for c, case in enumerate(cases):
+        n = case.n_points
+        href, pref = oracle_case(*b.initial[c], nsub)
+        assert rel_l2(b.force_history(c, nsub), href) <= tol_f, c
+        assert np.abs(b.points(c)[4:6, :n] - pref[4:6, :n]).max() <= tol_x, c
+    b.close()
