@@ -16,7 +16,7 @@ import pytest
 import oracle
 from icempm_amd import SimParams
 from icempm_amd._lib import BATCH_F32, BATCH_F64
-from icempm_amd.batch import batch_cases, build_batch
+from icempm_amd.batch import batch_cases, build_batch, ingest_cases
 from icempm_amd.learned import init_params, train_return_map
 
 pytestmark = pytest.mark.gpu
@@ -160,3 +160,46 @@ def test_trained_map_force_error_and_gain(gpu_available):
     rate = {k: v[0] / max(v[0] + v[1], 1) for k, v in rows.items()}
     assert rate[("trained", 1e-2)] > 0.5
     assert rate[("trained", 1e-2)] > 10 * rate[("untrained", 1e-2)]
+
+
+def test_train_on_50m_harvested_pairs(gpu_available):
+    """The harvest and training at full scale: 50M (trial, projected) pairs
+    harvested from exact FP64 batched runs of 32 cases (100k-500k points,
+    discs at 1 m/s into ice of 0.5 MPa tensile strength, so that many points
+    project), the network trained on the GPU on all of them, and a learned
+    FP32 run within 1% (indenter force, relative L2, every case) of the exact
+    FP64 run of the same cases."""
+    target = 50_000_000
+    cs = batch_cases(32, 100000, 500000, 2024, speed_range=(1.0, 1.0), tensile_range=(0.5e6, 0.5e6))
+    states = ingest_cases(cs)
+    hb = build_batch(cs, BATCH_F64, states)
+    hb.set_harvest(target)
+    total, runs = 0, 0
+    while total < target and runs < 30:
+        hb.run(2000)
+        runs += 1
+        _, _, total = hb.read_harvest(0)
+        print(f"harvest: {runs * 2000} substeps, {total} pairs offered", flush=True)
+    assert hb.error_flag() == 0
+    X, T, total = hb.read_harvest(target)
+    hb.close()
+    assert X.shape[0] == target
+    params, norm, losses = train_return_map(X, T, steps=3000, batch=8192, lr=3e-3, seed=0)
+    del X, T
+    print("trained on", target, "pairs, loss", losses[-1], flush=True)
+    assert losses[-1][1] < 0.1
+    nsub = 2000
+    ref = build_batch(cs, BATCH_F64, states)
+    ref.run(nsub)
+    ref_hist = histories(ref, nsub)
+    ref.close()
+    lb = build_batch(cs, BATCH_F32, states)
+    lb.set_learned(params, norm, tol=1e-4)
+    lb.run(nsub)
+    assert lb.error_flag() == 0
+    acc, fb, ins = lb.map_stats()
+    errs = [rel_l2(h, r) for h, r in zip(histories(lb, nsub), ref_hist)]
+    lb.close()
+    print(f"learned FP32: accepted {acc} fallback {fb} fallback rate {fb / max(acc + fb, 1):.4f} "
+          f"max force rel L2 {max(errs):.3e}", flush=True)
+    assert max(errs) <= 1e-2
