@@ -11,4 +11,8 @@
 
 #define ICEMPM_PASS_NS fast
 #define ICEMPM_FAST_ALGEBRA 1
+// six wavefronts per SIMD for the wave kernel: the compiler then fits it in
+// 80 VGPRs with a 12-byte spill, and the fused pass runs ~3% faster than at
+// 82 VGPRs and five wavefronts (measured on the 1.6M-point case)
+#define ICEMPM_WAVE_KERNEL_ATTR __attribute__((amdgpu_waves_per_eu(6)))
 #include "particles.inc"
