@@ -122,7 +122,7 @@ HIP_EXPORTS = [
     "icempm_transfer_points_to_device", "icempm_reset_grid",
     "icempm_reset_indenter_force_accumulator", "icempm_p2g", "icempm_update_nodes",
     "icempm_g2p", "icempm_run_substeps", "icempm_force_history", "icempm_transfer_from_device",
-    "icempm_error_code", "icempm_read_error_flag", "icempm_event_record_start",
+    "icempm_error_code", "icempm_transfer_wait", "icempm_read_error_flag", "icempm_event_record_start",
     "icempm_event_record_stop", "icempm_event_elapsed_ms", "icempm_sort_points",
     "icempm_sort_raw_points", "icempm_set_mode", "icempm_get_mode", "icempm_kernel_timing",
     "icempm_kernel_timing_read", "icempm_read_grid", "icempm_read_points",
@@ -130,7 +130,7 @@ HIP_EXPORTS = [
     "icempm_measure_copy", "icempm_batch_create", "icempm_batch_destroy",
     "icempm_batch_last_error", "icempm_batch_add_case", "icempm_batch_finalize",
     "icempm_batch_run", "icempm_batch_last_ms", "icempm_batch_num_points",
-    "icempm_batch_force_history", "icempm_batch_read_points", "icempm_batch_error_flag",
+    "icempm_batch_force_history", "icempm_batch_force_bins", "icempm_batch_read_points", "icempm_batch_error_flag",
     "icempm_batch_set_return_map", "icempm_batch_read_harvest", "icempm_batch_map_stats",
     "icempm_batch_num_nodes", "icempm_batch_kernel_timing", "icempm_batch_kernel_timing_read",
     "icempm_batch_set_tuning", "icempm_batch_get_tuning", "icempm_batch_resort",
@@ -188,6 +188,7 @@ def hip():
     _sig(L, "icempm_force_history", c_int, P, POINTER(c_double), c_int)
     _sig(L, "icempm_transfer_from_device", c_int, P, _CB, c_void_p)
     _sig(L, "icempm_error_code", c_int, P)
+    _sig(L, "icempm_transfer_wait", c_int, P)
     _sig(L, "icempm_read_error_flag", c_int, P, POINTER(c_int))
     _sig(L, "icempm_event_record_start", c_int, P)
     _sig(L, "icempm_event_record_stop", c_int, P)
@@ -215,6 +216,7 @@ def hip():
     _sig(L, "icempm_batch_last_ms", c_int, P, POINTER(c_float))
     _sig(L, "icempm_batch_num_points", c_int, P)
     _sig(L, "icempm_batch_force_history", c_int, P, c_int, POINTER(c_double), c_int)
+    _sig(L, "icempm_batch_force_bins", c_int, P, c_int, POINTER(c_double), c_int)
     _sig(L, "icempm_batch_read_points", c_int, P, c_int, POINTER(c_double), c_size_t)
     _sig(L, "icempm_batch_error_flag", c_int, P, POINTER(c_int))
     _sig(L, "icempm_batch_set_return_map", c_int, P, c_int, POINTER(c_float), POINTER(c_float),

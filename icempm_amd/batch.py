@@ -120,6 +120,14 @@ class Batch:
         self._check(self._H.icempm_batch_force_history(self._b, c, out.ctypes.data_as(POINTER(c_double)), n))
         return out
 
+    def force_bins(self, c: int) -> np.ndarray:
+        """Case c's indenter force bins (Indenter_2D: fx, fy per angular bin)
+        summed over the last run."""
+        nb = int(self.initial[c][0].n_indenter_subdivisions)
+        out = np.zeros(2 * nb)
+        self._check(self._H.icempm_batch_force_bins(self._b, c, out.ctypes.data_as(POINTER(c_double)), nb))
+        return out
+
     def points(self, c: int) -> np.ndarray:
         pitch = self.initial[c][1].shape[1]
         out = np.zeros((16, pitch))

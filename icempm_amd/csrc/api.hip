@@ -21,6 +21,8 @@ struct icempm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t evStart = nullptr, evStop = nullptr;
+    hipEvent_t evXfer = nullptr;   // recorded after the readback's copies
+    bool xferPending = false;
     std::string lastError;
 
     icy::SimParams prms{};
@@ -420,6 +422,7 @@ int icempm_create(icempm_ctx **out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->evStart);
     if (e == hipSuccess) e = hipEventCreate(&c->evStop);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evXfer, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c->dErr, sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc(&c->hostErr, sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = launch_fill_i32(c->dErr, 1, 0, c->stream);
@@ -441,6 +444,7 @@ void icempm_destroy(icempm_ctx *c) {
     for (hipEvent_t ev : c->tev) (void)hipEventDestroy(ev);
     if (c->evStart) (void)hipEventDestroy(c->evStart);
     if (c->evStop) (void)hipEventDestroy(c->evStop);
+    if (c->evXfer) (void)hipEventDestroy(c->evXfer);
     dfree(c->dErr);
     dfree(c->grainTab);
     hfree(c->hostErr);
@@ -688,12 +692,24 @@ int icempm_transfer_from_device(icempm_ctx *c, icempm_callback cb, void *user) {
                              hipMemcpyDeviceToHost, c->stream), "cuda_transfer_from_device");
     HIPCHK(c, hipMemcpyAsync(c->hostErr, c->dErr, sizeof(int), hipMemcpyDeviceToHost, c->stream),
            "cuda_transfer_from_device");
+    HIPCHK(c, hipEventRecord(c->evXfer, c->stream), "cuda_transfer_from_device");
+    c->xferPending = true;
     ReadbackJob *j = new ReadbackJob{c, cb, user};
     hipError_t e = hipLaunchHostFunc(c->stream, readback_done, j);
     if (e != hipSuccess) {
         delete j;
         return hipfail(c, e, "cuda_transfer_from_device");
     }
+    return ICEMPM_OK;
+}
+
+int icempm_transfer_wait(icempm_ctx *c) {
+    if (!c) return ICEMPM_E_ARG;
+    if (!c->xferPending) return ICEMPM_OK;
+    // the copies only: the completion callback queued after them may still run
+    HIPCHK(c, hipEventSynchronize(c->evXfer), "cuda_transfer_from_device");
+    c->xferPending = false;
+    c->errorCode.store(*c->hostErr);
     return ICEMPM_OK;
 }
 

@@ -6,18 +6,21 @@
 // Storage (per batch, cases contiguous, each case in band order):
 //   F64 variant    positions: 2 rows of double; vel(2) Fe(4) Bp(4) Jp_inv(1):
 //                  11 rows of double
-//   F32 variant    positions: 2 rows of 32-bit fixed point over the case's
-//                  grid width; vel(2) Fe-I(4) Bp(4) Jp_inv-1(1): 11 rows of
-//                  float.  Fe and Jp_inv are stored as their deviations from 1
-//                  so that float keeps them to ~1e-12 absolute.
+//   F32 variant    positions: 48-bit fixed point over the case's grid width,
+//                  3 rows of u32 (x high 32 bits, y high 32 bits, both low
+//                  16-bit halves); vel(2) Fe-I(4) Bp(4) Jp_inv-1(1): 11 rows
+//                  of float.  Fe and Jp_inv are stored as their deviations
+//                  from 1 so that float keeps them to ~1e-12 absolute.
 //   crushed        1 byte per point
+//   grain          i16 per point, only when a case has GrainVariability != 0
 //   grids          two grids of 3 rows of double per node (mass, momentum;
 //                  velocity after the update), cases concatenated, swapped
 //                  every substep; the F32 variant also keeps one FP32
 //                  velocity pair per node for the gathers
 // All arithmetic is FP64 in both variants (the return map included); the F32
 // variant only stores less: a fused pass moves 49 B per point instead of 97 B.
-// Material is uniform per case (one set of NACC constants per case).
+// A case's NACC strengths are uniform (constants per case) or vary per grain
+// id as GetParametersForGrain makes them (gpu_implementation5.cu:97-111).
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -39,18 +42,22 @@ namespace batch {
 struct CaseParams {
     double dt, vol, mass, h, h_inv, Dinv, mu, kappa, gravity, indRsq, indV;
     double ics, its, iss, tanPhi, dpThr, x0, y0;
-    double pmax, msq, betaPmax, onePlus2Beta, slope;   // NACC constants of the case's material
-    double posQ, posInvQ;   // FP32 storage: position quantum (grid width / 2^32) and its inverse
+    double pmax, msq, betaPmax, onePlus2Beta, slope;   // NACC constants of the case's material (grain 0)
+    double grainVar;        // GrainVariability: 0 = the constants above hold for every point
+    double posQ, posInvQ;   // FP32 storage: position quantum (grid width / 2^48) and its inverse
     double scm;             // dt V_p D^-1 / m_p: the stress term of the affine momentum per unit mass
-    int gx, gy, gridTotal, nPts, ptOffset, gridOffset, pad0, pad1;
+    int gx, gy, gridTotal, nPts, ptOffset, gridOffset;
+    int nbins, binOffset;   // indenter force bins: n_indenter_subdivisions, first double in the bin array
 };
 
 constexpr int kWX = 48, kWS = 52, kWY = 4;   // window width, padded row stride, rows
 
 // Point rows.  FP64 storage: positions FP64.  FP32 storage: positions as
-// 32-bit fixed point over the case's grid width (a quantum of gx h / 2^32,
-// 7e-10 m for a 3 m grid: a substep moves a point by ~1e-6 m, which an FP32
-// coordinate could not resolve), the other values FP32.
+// 48-bit fixed point over the case's grid width (a quantum of gx h / 2^48,
+// ~1e-14 m for a 3 m grid: a substep moves a point by ~1e-6 m, which an FP32
+// coordinate could not resolve, and 32 bits -- a quantum of ~7e-10 m --
+// rounded each displacement by ~1e-4 of itself, enough to move the force
+// history of a fracturing case by ~1%), the other values FP32.
 template <bool F32>
 struct Rows {
     using V = typename std::conditional<F32, float, double>::type;
@@ -58,24 +65,88 @@ struct Rows {
     // grid velocity as the gathers read it: FP64 rows 1-2 of the grid, or
     // (FP32 storage) one FP32 pair per node written by the grid update
     using G = typename std::conditional<F32, float2, double>::type;
-    P *pos;            // 2 rows
+    static constexpr int kPosRows = F32 ? 3 : 2;
+    P *pos;            // kPosRows rows
     V *rest;           // 11 rows: vel(2) Fe(4) Bp(4) Jp(1)
     unsigned char *crushed;
+    short *grain;      // null when every case is uniform
     size_t pitch;
 };
 
-// a stored coordinate as FP64, and back (round to nearest; outside the grid
-// and NaN go to the grid's edge, where the next stencil check flags them)
-static __host__ __device__ inline double pos_get(double v, const CaseParams &) { return v; }
-static __host__ __device__ inline double pos_get(unsigned int v, const CaseParams &cp) { return (double)v * cp.posQ; }
-template <class P>
-static __host__ __device__ inline P pos_put(double x, const CaseParams &cp);
-template <>
-__host__ __device__ inline double pos_put<double>(double x, const CaseParams &) { return x; }
-template <>
-__host__ __device__ inline unsigned int pos_put<unsigned int>(double x, const CaseParams &cp) {
-    const double u = rint(x * cp.posInvQ);
-    return (u >= 0.0 && u <= 4294967295.0) ? (unsigned int)u : (u < 0.0 ? 0u : 4294967295u);
+constexpr double kFix48Max = 281474976710655.0;   // 2^48 - 1
+
+// FP32 storage: a coordinate as 48-bit fixed point, u = hi 2^16 + lo
+// (round to nearest; outside [0, gx h) and NaN go to the grid's edge, where
+// the next stencil check flags them).  Every step is exact in FP64 except
+// the scalings by posQ / posInvQ.
+static __host__ __device__ inline double fix48_get(unsigned int hi, unsigned int lo, const CaseParams &cp) {
+    return ((double)hi * 65536.0 + (double)lo) * cp.posQ;
+}
+static __host__ __device__ inline void fix48_put(double x, const CaseParams &cp, unsigned int &hi, unsigned int &lo) {
+    double u = rint(x * cp.posInvQ);
+    u = (u >= 0.0 && u <= kFix48Max) ? u : (u < 0.0 ? 0.0 : kFix48Max);
+    const double uh = floor(u * (1.0 / 65536.0));
+    hi = (unsigned int)uh;
+    lo = (unsigned int)(u - uh * 65536.0);
+}
+// whether a coordinate lies where 48-bit fixed point represents it
+static __host__ __device__ inline bool fix48_in_range(double x, const CaseParams &cp) {
+    const double u = x * cp.posInvQ;
+    return u >= 0.0 && u < 281474976710656.0;
+}
+
+// Point position of slot pt (both storage formats) as FP64; store it back,
+// and the value the stored form holds.
+template <bool NT>
+static __device__ __forceinline__ void pos_load(const double *pos, size_t pp, int pt, const CaseParams &,
+                                                double &x, double &y) {
+    if constexpr (NT) {
+        x = __builtin_nontemporal_load(&pos[pt]);
+        y = __builtin_nontemporal_load(&pos[pp + pt]);
+    } else {
+        x = pos[pt];
+        y = pos[pp + pt];
+    }
+}
+template <bool NT>
+static __device__ __forceinline__ void pos_load(const unsigned int *pos, size_t pp, int pt, const CaseParams &cp,
+                                                double &x, double &y) {
+    unsigned int hx, hy, lo;
+    if constexpr (NT) {
+        hx = __builtin_nontemporal_load(&pos[pt]);
+        hy = __builtin_nontemporal_load(&pos[pp + pt]);
+        lo = __builtin_nontemporal_load(&pos[2 * pp + pt]);
+    } else {
+        hx = pos[pt];
+        hy = pos[pp + pt];
+        lo = pos[2 * pp + pt];
+    }
+    x = fix48_get(hx, lo & 0xFFFFu, cp);
+    y = fix48_get(hy, lo >> 16, cp);
+}
+// stores (x, y) and replaces them by the values the rows now hold
+static __device__ __forceinline__ void pos_store(double *pos, size_t pp, int pt, const CaseParams &, double &x,
+                                                 double &y) {
+    __builtin_nontemporal_store(x, &pos[pt]);
+    __builtin_nontemporal_store(y, &pos[pp + pt]);
+}
+static __device__ __forceinline__ void pos_store(unsigned int *pos, size_t pp, int pt, const CaseParams &cp,
+                                                 double &x, double &y) {
+    unsigned int hx, lx, hy, ly;
+    fix48_put(x, cp, hx, lx);
+    fix48_put(y, cp, hy, ly);
+    __builtin_nontemporal_store(hx, &pos[pt]);
+    __builtin_nontemporal_store(hy, &pos[pp + pt]);
+    __builtin_nontemporal_store(lx | (ly << 16), &pos[2 * pp + pt]);
+    x = fix48_get(hx, lx, cp);
+    y = fix48_get(hy, ly, cp);
+}
+// the value a coordinate takes when stored (without storing it)
+static __host__ __device__ inline double pos_round(double x, const CaseParams &, double) { return x; }
+static __host__ __device__ inline double pos_round(double x, const CaseParams &cp, unsigned int) {
+    unsigned int h, l;
+    fix48_put(x, cp, h, l);
+    return fix48_get(h, l, cp);
 }
 
 // the velocity of a node (index within its case's grid) as the gathers read it
@@ -121,6 +192,7 @@ static __device__ __forceinline__ KParams kp_of(const CaseParams &c, int *err) {
     k.dt = c.dt; k.vol = c.vol; k.mass = c.mass; k.h = c.h; k.h_inv = c.h_inv; k.Dinv = c.Dinv;
     k.mu = c.mu; k.kappa = c.kappa; k.gravity = c.gravity; k.indRsq = c.indRsq; k.indV = c.indV;
     k.ics = c.ics; k.its = c.its; k.iss = c.iss; k.tanPhi = c.tanPhi; k.dpThr = c.dpThr;
+    k.grainVar = c.grainVar;
     return k;
 }
 
@@ -226,8 +298,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                                                          typename Rows<F32>::V *rest, unsigned char *crushedRow,
                                                          size_t pitch, const typename Rows<F32>::G *__restrict__ gridV,
                                                          double *__restrict__ gridOut, size_t gpitch,
-                                                         int *__restrict__ err, LearnArgs la, MapQueue mq) {
-    const Rows<F32> S{pos, rest, crushedRow, pitch};
+                                                         int *__restrict__ err, const short *__restrict__ grains,
+                                                         LearnArgs la, MapQueue mq) {
+    const Rows<F32> S{pos, rest, crushedRow, nullptr, pitch};
     __shared__ __attribute__((aligned(16))) double smem[4][kWaveWinBytes / 8];
     const int per = (nChunks + 7) >> 3;
     const int c = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
@@ -259,15 +332,16 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
     if constexpr (G2P) {
         double Jp = 1;
         unsigned char crushed = 0;
+        short grain = 0;
         if (valid) {
             // streamed once per substep: non-temporal, so the grid keeps the L2
-            px = pos_get(__builtin_nontemporal_load(&S.pos[pt]), cp);
-            py = pos_get(__builtin_nontemporal_load(&S.pos[pp + pt]), cp);
+            pos_load<true>(S.pos, pp, pt, cp, px, py);
             F00 = fe0 + (double)__builtin_nontemporal_load(&R[pp * 2 + pt]);
             F01 = (double)__builtin_nontemporal_load(&R[pp * 3 + pt]);
             F10 = (double)__builtin_nontemporal_load(&R[pp * 4 + pt]);
             F11 = fe0 + (double)__builtin_nontemporal_load(&R[pp * 5 + pt]);
             crushed = __builtin_nontemporal_load(&S.crushed[pt]);
+            if (cp.grainVar != 0) grain = __builtin_nontemporal_load(&grains[pt]);
         }
         int bx = 0, by = 0;
         double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
@@ -349,7 +423,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             Jp0 = Jp;
             if (crushed == 0) {
                 bool f;
-                if (iv.p < 0) {
+                if (cp.grainVar != 0) {   // GetParametersForGrain's strengths (case-uniform branch)
+                    f = fails(k, grain, iv.p, iv.q);
+                } else if (iv.p < 0) {
                     f = (iv.p < -3e6) || (iv.q > cp.slope * (iv.p - -3e6));
                 } else {
                     f = cp.onePlus2Beta * iv.q * iv.q + cp.msq * (iv.p + cp.betaPmax) * (iv.p - cp.pmax) > 0;
@@ -428,8 +504,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                         e.Jp = Jp;
                         // the position at its stored precision, as the P2G here would take it
                         using PT = typename Rows<F32>::P;
-                        e.x[0] = pos_get(pos_put<PT>(px, cp), cp);
-                        e.x[1] = pos_get(pos_put<PT>(py, cp), cp);
+                        e.x[0] = pos_round(px, cp, PT{});
+                        e.x[1] = pos_round(py, cp, PT{});
                         e.v[0] = vx; e.v[1] = vy;
                         e.B[0] = B00; e.B[1] = B01; e.B[2] = B10; e.B[3] = B11;
                     } else {
@@ -448,16 +524,14 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             fresh = ok && crushed == 0;
         }
         if (ok) {
-            using PT = typename Rows<F32>::P;
-            const PT qx = pos_put<PT>(px, cp), qy = pos_put<PT>(py, cp);
             if constexpr (F32) {
                 // fixed point clamps to [0, gx h): a position outside (or
                 // NaN) is an error here, as the FP64 rows would report it
-                const double ux = px * cp.posInvQ, uy = py * cp.posInvQ;
-                if (!(ux >= 0.0 && ux < 4294967296.0 && uy >= 0.0 && uy < 4294967296.0)) flag_error(k);
+                if (!(fix48_in_range(px, cp) && fix48_in_range(py, cp))) flag_error(k);
             }
-            __builtin_nontemporal_store(qx, &S.pos[pt]);
-            __builtin_nontemporal_store(qy, &S.pos[pp + pt]);
+            // px, py become what the rows hold: the P2G takes them at their
+            // stored precision
+            pos_store(S.pos, pp, pt, cp, px, py);
             if (!defer) {
                 __builtin_nontemporal_store((V)(F00 - fe0), &R[pp * 2 + pt]);
                 __builtin_nontemporal_store((V)F01, &R[pp * 3 + pt]);
@@ -479,11 +553,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             if (Jp != Jp0 && !defer) R[pp * 10 + pt] = (V)(Jp - fe0);
             if (crushed != crushed0) S.crushed[pt] = crushed;
             if constexpr (F32) {
-                // position and Fe are what the next substep reads back: the
-                // P2G takes them at their stored precision.  v and Bp stay
-                // FP64: a fused pass does not store them, and the next G2P
-                // recomputes them.
-                px = pos_get(qx, cp); py = pos_get(qy, cp);
+                // Fe is what the next substep reads back: the P2G takes it at
+                // its stored precision.  v and Bp stay FP64: a fused pass
+                // does not store them, and the next G2P recomputes them.
                 F00 = fe0 + (double)(V)(F00 - fe0); F01 = (double)(V)F01;
                 F10 = (double)(V)F10; F11 = fe0 + (double)(V)(F11 - fe0);
             }
@@ -494,8 +566,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         bool have = valid;
         if constexpr (!G2P) {
             if (valid) {
-                px = pos_get(S.pos[pt], cp);
-                py = pos_get(S.pos[pp + pt], cp);
+                pos_load<false>(S.pos, pp, pt, cp, px, py);
                 vx = (double)R[pt];
                 vy = (double)R[pp + pt];
                 F00 = fe0 + (double)R[pp * 2 + pt];
@@ -796,7 +867,7 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
                                                      double *__restrict__ hist, float2 *__restrict__ vel32,
                                                      int *__restrict__ gridHasMass,
                                                      const int *__restrict__ zeroHasMass,
-                                                     int *__restrict__ velNonzero) {
+                                                     int *__restrict__ velNonzero, double *__restrict__ bins) {
     __shared__ double sf[2];
     __shared__ int any;
     const int2 nb = nodeBlock[blockIdx.x];
@@ -853,9 +924,20 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
                     const double p0 = v0, p1 = v1;
                     v0 = V + (r0 - n0 * vnr);
                     v1 = r1 - n1 * vnr;
-                    atomicAdd(&sf[0], (p0 - v0) * mass / dt);
-                    atomicAdd(&sf[1], (p1 - v1) * mass / dt);
+                    const double f0 = (p0 - v0) * mass / dt, f1 = (p1 - v1) * mass / dt;
+                    atomicAdd(&sf[0], f0);
+                    atomicAdd(&sf[1], f1);
                     any = 1;
+                    // the force's angular bin (gpu_implementation5.cu:262-269:
+                    // atan2 of (x, y) of the unit normal, then + pi, scaled)
+                    double angle = atan2(n0, n1);
+                    angle += kPi;
+                    angle *= cp.nbins / (2 * kPi);
+                    int kb = (int)angle;
+                    kb = max(kb, 0);
+                    kb = min(kb, cp.nbins - 1);
+                    unsafeAtomicAdd(&bins[cp.binOffset + 2 * kb], f0);
+                    unsafeAtomicAdd(&bins[cp.binOffset + 2 * kb + 1], f1);
                 }
             }
             if (iy <= 2) {
@@ -897,7 +979,8 @@ __global__ void k_batch_keys(const PT *__restrict__ pos, size_t pitch, int nPts,
         if (cases[mid].ptOffset <= i) lo = mid; else hi = mid - 1;
     }
     const CaseParams &cp = cases[lo];
-    const double x = pos_get(pos[i], cp), y = pos_get(pos[pitch + i], cp);
+    double x, y;
+    pos_load<false>(pos, pitch, i, cp, x, y);
     int key = cp.gridTotal - 1;
     if (x == x && y == y) {
         const double bx = fmin(fmax(x * cp.h_inv - 0.5, 0.0), (double)(cp.gx - 1));
@@ -916,10 +999,10 @@ __global__ void k_batch_permute(Rows<F32> src, Rows<F32> dst, const int *__restr
     if (k >= nPts) return;
     const int j = perm[k];
     const size_t p = src.pitch;
-    dst.pos[k] = src.pos[j];
-    dst.pos[p + k] = src.pos[p + j];
+    for (int r = 0; r < Rows<F32>::kPosRows; ++r) dst.pos[r * p + k] = src.pos[r * p + j];
     for (int r = 0; r < 11; ++r) dst.rest[r * p + k] = src.rest[r * p + j];
     dst.crushed[k] = src.crushed[j];
+    if (src.grain) dst.grain[k] = src.grain[j];
     dstOrig[k] = srcOrig[j];
 }
 
@@ -940,14 +1023,18 @@ struct icempm_batch {
         std::vector<double> pos;        // 2 x n
         std::vector<double> rest;       // 11 x n
         std::vector<unsigned char> crushed;
+        std::vector<short> grain;
         std::vector<int> keys;
     };
     std::vector<HostCase> hc;
 
     // device
-    void *pos = nullptr;          // 2 rows of Rows<F32>::P
+    void *pos = nullptr;          // Rows<F32>::kPosRows rows of Rows<F32>::P
     void *rest = nullptr;
     unsigned char *crushed = nullptr;
+    short *grain = nullptr;       // per-point grain ids (when a case has GrainVariability != 0)
+    double *bins = nullptr;       // indenter force bins of every case, summed over a run
+    int binsTotal = 0;            // doubles
     size_t pitch = 0;
     int nPts = 0;
     double *gridA = nullptr, *gridB = nullptr;
@@ -1001,12 +1088,14 @@ struct icempm_batch {
     void *tpos = nullptr;
     void *trest = nullptr;
     unsigned char *tcrushed = nullptr;
+    short *tgrain = nullptr;
     int *torig = nullptr;
     // checkpoint of the state
     bool haveCheckpoint = false;
     void *kpos = nullptr;
     void *krest = nullptr;
     unsigned char *kcrushed = nullptr;
+    short *kgrain = nullptr;
     int *korig = nullptr;
     double kt = 0;
     int kstep = 0;
@@ -1048,11 +1137,11 @@ template <bool F32, bool G2P, bool P2G, int RMAP>
 void launch_one(icempm_batch *b, unsigned nb, const void *gV, double *gO, const LearnArgs &la,
                 const MapQueue &mq) {
     Rows<F32> S{static_cast<typename Rows<F32>::P *>(b->pos), reinterpret_cast<typename Rows<F32>::V *>(b->rest),
-                b->crushed, b->pitch};
+                b->crushed, b->grain, b->pitch};
     hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream,
                        b->chunks, b->nChunks, b->cases, S.pos, S.rest, S.crushed, S.pitch,
                        static_cast<const typename Rows<F32>::G *>(gV), gO, b->gpitch,
-                       b->err, la, mq);
+                       b->err, b->grain, la, mq);
     if (b->evAfter) (void)hipEventRecord(b->evAfter, b->stream);
     if constexpr (G2P && RMAP == kMapLearned) {
         // the deferred points: map, store, and their P2G into the pass's output grid
@@ -1159,6 +1248,8 @@ int upload_chunks(icempm_batch *b) {
     const size_t n = b->hccase.size();
     std::vector<int4> t(std::max<size_t>(n, 1));
     for (size_t c = 0; c < n; ++c) t[c] = make_int4(b->hcstart[c], b->hcstart[c + 1], b->hccase[c], 0);
+    const int4 *before = b->chunks;
+    const int nBefore = b->nChunks;
     if (t.size() > b->chunkCapDev) {
         dfree(b->chunks);
         b->chunkCapDev = 0;
@@ -1167,7 +1258,9 @@ int upload_chunks(icempm_batch *b) {
     }
     BCHK(b, hipMemcpy(b->chunks, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice), "chunk table");
     b->nChunks = (int)n;
-    drop_graph(b);
+    // a captured loop holds the table's address and the chunk count (the
+    // grid size of its launches); the table's contents are read at run time
+    if (b->chunks != before || b->nChunks != nBefore) drop_graph(b);
     return ICEMPM_OK;
 }
 
@@ -1199,7 +1292,8 @@ int current_keys(icempm_batch *b, std::vector<int> &local) {
 }
 
 size_t pos_bytes(const icempm_batch *b) {
-    return 2 * b->pitch * (b->precision == ICEMPM_BATCH_F32 ? sizeof(unsigned int) : sizeof(double));
+    return b->precision == ICEMPM_BATCH_F32 ? Rows<true>::kPosRows * b->pitch * sizeof(unsigned int)
+                                             : Rows<false>::kPosRows * b->pitch * sizeof(double);
 }
 
 size_t rest_bytes(const icempm_batch *b) {
@@ -1223,23 +1317,30 @@ int resort(icempm_batch *b) {
         BCHK(b, hipMalloc(&b->trest, rest_bytes(b)), "re-sort");
         BCHK(b, hipMalloc(&b->tcrushed, b->pitch), "re-sort");
         BCHK(b, hipMalloc(&b->torig, b->pitch * sizeof(int)), "re-sort");
+        if (b->grain) BCHK(b, hipMalloc(&b->tgrain, b->pitch * sizeof(short)), "re-sort");
     }
     BCHK(b, radix_sort_pairs(b->sKeys, b->sVals, b->sKeys2, b->sVals2, (size_t)n, b->sTmp, b->stream), "re-sort");
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (b->precision == ICEMPM_BATCH_F32) {
-        Rows<true> a{static_cast<unsigned int *>(b->pos), reinterpret_cast<float *>(b->rest), b->crushed, b->pitch};
-        Rows<true> t{static_cast<unsigned int *>(b->tpos), reinterpret_cast<float *>(b->trest), b->tcrushed, b->pitch};
+        Rows<true> a{static_cast<unsigned int *>(b->pos), reinterpret_cast<float *>(b->rest), b->crushed, b->grain,
+                     b->pitch};
+        Rows<true> t{static_cast<unsigned int *>(b->tpos), reinterpret_cast<float *>(b->trest), b->tcrushed,
+                     b->tgrain, b->pitch};
         hipLaunchKernelGGL(k_batch_permute<true>, dim3(nb), dim3(256), 0, b->stream, a, t, b->orig, b->torig, b->sVals, n);
     } else {
-        Rows<false> a{static_cast<double *>(b->pos), reinterpret_cast<double *>(b->rest), b->crushed, b->pitch};
-        Rows<false> t{static_cast<double *>(b->tpos), reinterpret_cast<double *>(b->trest), b->tcrushed, b->pitch};
+        Rows<false> a{static_cast<double *>(b->pos), reinterpret_cast<double *>(b->rest), b->crushed, b->grain,
+                      b->pitch};
+        Rows<false> t{static_cast<double *>(b->tpos), reinterpret_cast<double *>(b->trest), b->tcrushed,
+                      b->tgrain, b->pitch};
         hipLaunchKernelGGL(k_batch_permute<false>, dim3(nb), dim3(256), 0, b->stream, a, t, b->orig, b->torig, b->sVals, n);
     }
     BCHK(b, hipGetLastError(), "re-sort");
     std::swap(b->pos, b->tpos);
     std::swap(b->rest, b->trest);
     std::swap(b->crushed, b->tcrushed);
+    std::swap(b->grain, b->tgrain);
     std::swap(b->orig, b->torig);
+    drop_graph(b);   // the captured launches hold the old row addresses
     // the sorted keys give the new chunk table
     BCHK(b, hipMemcpyAsync(local.data(), b->sKeys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->stream), "re-sort");
     BCHK(b, hipStreamSynchronize(b->stream), "re-sort");
@@ -1279,7 +1380,7 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
         const bool last = (i == n - 1);
         hipLaunchKernelGGL(k_batch_update, dim3(b->nNodeBlocks), dim3(256), 0, b->stream, b->cases,
                            b->nodeBlock, A, last ? nullptr : B, b->gpitch, tTab, i,
-                           hist + 2 * nc * (size_t)i, b->vel32, fA, fB, fV);
+                           hist + 2 * nc * (size_t)i, b->vel32, fA, fB, fV, b->bins);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         const void *gV = b->vel32 ? static_cast<const void *>(b->vel32) : static_cast<const void *>(A);
         if (!last) {
@@ -1345,6 +1446,7 @@ void icempm_batch_destroy(icempm_batch *b) {
     dfree(b->orig); dfree(b->sKeys); dfree(b->sVals); dfree(b->sKeys2); dfree(b->sVals2); dfree(b->sTmp);
     dfree(b->tpos); dfree(b->trest); dfree(b->tcrushed); dfree(b->torig);
     dfree(b->kpos); dfree(b->krest); dfree(b->kcrushed); dfree(b->korig);
+    dfree(b->grain); dfree(b->tgrain); dfree(b->kgrain); dfree(b->bins);
     for (hipEvent_t ev : b->tev) (void)hipEventDestroy(ev);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
@@ -1364,8 +1466,19 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
         return -bfail(b, ICEMPM_E_ARG, "add_case: bad sizes");
     if (P.GridX > 32767 || P.GridY > 32767)   // cell indices travel as 16-bit halves in the particle pass
         return -bfail(b, ICEMPM_E_ARG, "add_case: grid wider or taller than 32767 cells");
-    if (P.GrainVariability != 0)
-        return -bfail(b, ICEMPM_E_ARG, "add_case: a batched case needs a uniform material (GrainVariability = 0)");
+    // the case's grid slice, its node blocks and every node index the kernels
+    // form (X + Y * GridX) are sized from GridTotal: it must be the product
+    if ((long)P.GridTotal != (long)P.GridX * (long)P.GridY)
+        return -bfail(b, ICEMPM_E_ARG, "add_case: GridTotal is not GridX * GridY");
+    // the stencils use cellsize_inv, the weights and stresses cellsize and dt
+    // (derived as in ComputeHelperVariables, parameters_sim.cpp:131-133)
+    if (!(P.cellsize > 0) || !(P.InitialTimeStep > 0) || !(P.cellsize_inv == 1. / P.cellsize) ||
+        !(P.Dp_inv == 4. / (P.cellsize * P.cellsize)))
+        return -bfail(b, ICEMPM_E_ARG, "add_case: cellsize, cellsize_inv, Dp_inv and dt must be positive and consistent");
+    if (!(P.GrainVariability == P.GrainVariability))
+        return -bfail(b, ICEMPM_E_ARG, "add_case: GrainVariability is NaN");
+    if (P.n_indenter_subdivisions <= 0 || P.n_indenter_subdivisions > 1 << 16)
+        return -bfail(b, ICEMPM_E_ARG, "add_case: n_indenter_subdivisions out of range");
     if (!b->hc.empty() && P.InitialTimeStep != b->hc[0].cp.dt)
         return -bfail(b, ICEMPM_E_ARG, "add_case: all cases of a batch share one time step");
     icempm_batch::HostCase h{};
@@ -1387,13 +1500,17 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
         c.slope = -(2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin)) / -3e6;
     }
     c.gx = P.GridX; c.gy = P.GridY; c.gridTotal = P.GridTotal; c.nPts = P.nPts;
+    c.grainVar = P.GrainVariability;
+    c.nbins = P.n_indenter_subdivisions;
+    c.binOffset = 0;   // set by finalize
     c.scm = c.dt * c.vol * c.Dinv / c.mass;
-    c.posQ = (c.gx * c.h) / 4294967296.0;   // FP32 storage: positions in [0, gx h) on 32 bits
+    c.posQ = (c.gx * c.h) / 281474976710656.0;   // FP32 storage: positions in [0, gx h) on 48 bits
     c.posInvQ = 1.0 / c.posQ;
     const int n = P.nPts;
     h.pos.resize(2 * (size_t)n);
     h.rest.resize(11 * (size_t)n);
     h.crushed.resize(n);
+    h.grain.resize(n);
     h.keys.resize(n);
     const unsigned char *u8 = reinterpret_cast<const unsigned char *>(soa);
     for (int i = 0; i < n; ++i) {
@@ -1402,6 +1519,7 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
         for (int r = 0; r < 10; ++r) h.rest[(size_t)r * n + i] = soa[pitch * (6 + r) + i];
         h.rest[10 * (size_t)n + i] = soa[pitch * 3 + i];
         h.crushed[i] = u8[i];
+        std::memcpy(&h.grain[i], u8 + pitch + 2 * (size_t)i, sizeof(short));   // point.cpp:19-23
         // the base cell's band key, clamped into the grid as on the device
         const double x = h.pos[i], y = h.pos[n + i];
         int key = c.gridTotal - 1;
@@ -1422,7 +1540,8 @@ int icempm_batch_finalize(icempm_batch *b) {
     // point and node offsets, node-block table, keys of the points in slot order
     std::vector<int2> nblk;
     std::vector<int> keys;
-    long pofs = 0, gofs = 0;
+    long pofs = 0, gofs = 0, bofs = 0;
+    bool anyGrains = false;
     // every case's points in band order (stable): slot o + k holds point
     // order[k] of the case, and orig maps it back
     std::vector<std::vector<int>> order(b->hc.size());
@@ -1430,6 +1549,9 @@ int icempm_batch_finalize(icempm_batch *b) {
         auto &h = b->hc[ci];
         h.cp.ptOffset = (int)pofs;
         h.cp.gridOffset = (int)gofs;
+        h.cp.binOffset = (int)bofs;
+        bofs += 2 * (long)h.cp.nbins;
+        anyGrains = anyGrains || h.cp.grainVar != 0;
         auto &ord = order[ci];
         ord.resize((size_t)h.cp.nPts);
         for (int i = 0; i < h.cp.nPts; ++i) ord[(size_t)i] = i;
@@ -1439,7 +1561,9 @@ int icempm_batch_finalize(icempm_batch *b) {
         pofs += h.cp.nPts;
         gofs += h.cp.gridTotal;
     }
-    if (pofs > INT_MAX / 2 || gofs > INT_MAX / 2) return bfail(b, ICEMPM_E_ARG, "finalize: batch too large");
+    if (pofs > INT_MAX / 2 || gofs > INT_MAX / 2 || bofs > INT_MAX / 2)
+        return bfail(b, ICEMPM_E_ARG, "finalize: batch too large");
+    b->binsTotal = (int)bofs;
     b->nPts = (int)pofs;
     b->nNodes = gofs;
     b->nNodeBlocks = (int)nblk.size();
@@ -1450,6 +1574,8 @@ int icempm_batch_finalize(icempm_batch *b) {
     BCHK(b, hipMalloc(&b->pos, pos_bytes(b)), "batch finalize");
     BCHK(b, hipMalloc(&b->rest, 11 * b->pitch * vs), "batch finalize");
     BCHK(b, hipMalloc(&b->crushed, b->pitch), "batch finalize");
+    if (anyGrains) BCHK(b, hipMalloc(&b->grain, b->pitch * sizeof(short)), "batch finalize");
+    BCHK(b, hipMalloc(&b->bins, (size_t)b->binsTotal * sizeof(double)), "batch finalize");
     BCHK(b, hipMalloc(&b->orig, b->pitch * sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->gridA, 3 * b->gpitch * sizeof(double)), "batch finalize");
     BCHK(b, hipMalloc(&b->gridB, 3 * b->gpitch * sizeof(double)), "batch finalize");
@@ -1464,6 +1590,7 @@ int icempm_batch_finalize(icempm_batch *b) {
     // host staging in the device layout
     std::vector<double> hpos(2 * b->pitch, 0.0);
     std::vector<unsigned char> hrest(11 * b->pitch * vs, 0), hcr(b->pitch, 0);
+    std::vector<short> hgr(b->pitch, 0);
     std::vector<int> horig(b->pitch, 0);
     std::vector<CaseParams> hcp;
     for (size_t ci = 0; ci < b->hc.size(); ++ci) {
@@ -1474,6 +1601,7 @@ int icempm_batch_finalize(icempm_batch *b) {
             hpos[o + k] = h.pos[i];
             hpos[b->pitch + o + k] = h.pos[n + i];
             hcr[o + k] = h.crushed[i];
+            hgr[o + k] = h.grain[i];
             horig[o + k] = o + i;
             for (int r = 0; r < 11; ++r) {
                 double v = h.rest[(size_t)r * n + i];
@@ -1489,14 +1617,19 @@ int icempm_batch_finalize(icempm_batch *b) {
         hcp.push_back(h.cp);
     }
     if (b->precision == ICEMPM_BATCH_F32) {
-        std::vector<unsigned int> q(hpos.size(), 0u);
+        // 48-bit fixed point: high words of x and y, then both low halves
+        std::vector<unsigned int> q(3 * b->pitch, 0u);
         for (size_t ci = 0; ci < hcp.size(); ++ci) {
             const CaseParams &cp = hcp[ci];
-            for (int k = 0; k < cp.nPts; ++k)
-                for (int a = 0; a < 2; ++a) {
-                    const size_t i = (size_t)a * b->pitch + cp.ptOffset + k;
-                    q[i] = pos_put<unsigned int>(hpos[i], cp);
-                }
+            for (int k = 0; k < cp.nPts; ++k) {
+                const size_t i = (size_t)cp.ptOffset + k;
+                unsigned int hx, lx, hy, ly;
+                fix48_put(hpos[i], cp, hx, lx);
+                fix48_put(hpos[b->pitch + i], cp, hy, ly);
+                q[i] = hx;
+                q[b->pitch + i] = hy;
+                q[2 * b->pitch + i] = lx | (ly << 16);
+            }
         }
         BCHK(b, hipMemcpy(b->pos, q.data(), q.size() * sizeof(unsigned int), hipMemcpyHostToDevice), "batch upload");
     } else {
@@ -1504,6 +1637,12 @@ int icempm_batch_finalize(icempm_batch *b) {
     }
     BCHK(b, hipMemcpy(b->rest, hrest.data(), hrest.size(), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->crushed, hcr.data(), hcr.size(), hipMemcpyHostToDevice), "batch upload");
+    if (b->grain)
+        BCHK(b, hipMemcpy(b->grain, hgr.data(), hgr.size() * sizeof(short), hipMemcpyHostToDevice), "batch upload");
+    {
+        const std::vector<double> zb((size_t)b->binsTotal, 0.0);
+        BCHK(b, hipMemcpy(b->bins, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice), "batch upload");
+    }
     BCHK(b, hipMemcpy(b->orig, horig.data(), horig.size() * sizeof(int), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->cases, hcp.data(), hcp.size() * sizeof(CaseParams), hipMemcpyHostToDevice), "batch upload");
     BCHK(b, hipMemcpy(b->nodeBlock, nblk.data(), nblk.size() * sizeof(int2), hipMemcpyHostToDevice), "batch upload");
@@ -1589,6 +1728,8 @@ int icempm_batch_run(icempm_batch *b, int n, int use_graph) {
         tt[i] = t;
     }
     BCHK(b, hipMemcpyAsync(b->tTab, tt.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream), "batch run");
+    // the force bins sum over the run (Model::Step sums them over a cycle, model.cpp:18)
+    BCHK(b, launch_fill_f64(b->bins, (size_t)b->binsTotal, 0.0, b->stream), "batch run");
     BCHK(b, hipStreamSynchronize(b->stream), "batch run");
     // segments of resortEvery substeps, the points re-sorted between them
     const int seg = b->resortEvery > 0 ? b->resortEvery : n;
@@ -1654,6 +1795,17 @@ int icempm_batch_force_history(icempm_batch *b, int c, double *out, int n) {
     return ICEMPM_OK;
 }
 
+int icempm_batch_force_bins(icempm_batch *b, int c, double *out, int nbins) {
+    if (!b || !out || c < 0 || (size_t)c >= b->hc.size()) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "force_bins: batch not finalized");
+    const CaseParams &cp = b->hc[c].cp;
+    if (nbins != cp.nbins) return bfail(b, ICEMPM_E_ARG, "force_bins: nbins is not the case's n_indenter_subdivisions");
+    BCHK(b, hipStreamSynchronize(b->stream), "force_bins");
+    BCHK(b, hipMemcpy(out, b->bins + cp.binOffset, 2 * (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost),
+         "force_bins");
+    return ICEMPM_OK;
+}
+
 int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) {
     if (!b || !soa || c < 0 || (size_t)c >= b->hc.size()) return ICEMPM_E_ARG;
     const CaseParams &cp = b->hc[c].cp;
@@ -1663,12 +1815,16 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
     std::vector<double> hpos(2 * (size_t)n);
     std::vector<unsigned char> hrest(11 * (size_t)n * vs), hcr(n);
     if (b->precision == ICEMPM_BATCH_F32) {
-        std::vector<unsigned int> q(2 * (size_t)n);
+        std::vector<unsigned int> q(3 * (size_t)n);
         const unsigned int *P = static_cast<const unsigned int *>(b->pos);
-        BCHK(b, hipMemcpy(q.data(), P + o, n * sizeof(unsigned int), hipMemcpyDeviceToHost), "read_points");
-        BCHK(b, hipMemcpy(q.data() + n, P + b->pitch + o, n * sizeof(unsigned int), hipMemcpyDeviceToHost),
-             "read_points");
-        for (size_t i = 0; i < q.size(); ++i) hpos[i] = pos_get(q[i], cp);
+        for (int r = 0; r < 3; ++r)
+            BCHK(b, hipMemcpy(q.data() + (size_t)r * n, P + r * b->pitch + o, n * sizeof(unsigned int),
+                              hipMemcpyDeviceToHost), "read_points");
+        for (int i = 0; i < n; ++i) {
+            const unsigned int lo = q[2 * (size_t)n + i];
+            hpos[i] = fix48_get(q[i], lo & 0xFFFFu, cp);
+            hpos[n + i] = fix48_get(q[(size_t)n + i], lo >> 16, cp);
+        }
     } else {
         const double *P = static_cast<const double *>(b->pos);
         BCHK(b, hipMemcpy(hpos.data(), P + o, n * sizeof(double), hipMemcpyDeviceToHost), "read_points");
@@ -1679,6 +1835,8 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
         BCHK(b, hipMemcpy(hrest.data() + (size_t)r * n * vs, (unsigned char *)b->rest + ((size_t)r * b->pitch + o) * vs,
                           n * vs, hipMemcpyDeviceToHost), "read_points");
     BCHK(b, hipMemcpy(hcr.data(), b->crushed + o, n, hipMemcpyDeviceToHost), "read_points");
+    std::vector<short> hgr((size_t)n, 0);
+    if (b->grain) BCHK(b, hipMemcpy(hgr.data(), b->grain + o, n * sizeof(short), hipMemcpyDeviceToHost), "read_points");
     std::vector<int> horig((size_t)n);
     BCHK(b, hipMemcpy(horig.data(), b->orig + o, n * sizeof(int), hipMemcpyDeviceToHost), "read_points");
     std::memset(soa, 0, 16 * pitch * sizeof(double));
@@ -1688,6 +1846,7 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
         const int i = horig[slot] - o;
         if (i < 0 || i >= n) return bfail(b, ICEMPM_E_STATE, "read_points: slot map corrupt");
         u8[i] = hcr[slot];
+        std::memcpy(u8 + pitch + 2 * (size_t)i, &hgr[slot], sizeof(short));   // point.cpp:19-23
         soa[pitch * 4 + i] = hpos[slot];
         soa[pitch * 5 + i] = hpos[n + slot];
         for (int r = 0; r < 11; ++r) {
@@ -1831,11 +1990,15 @@ int icempm_batch_checkpoint(icempm_batch *b) {
         BCHK(b, hipMalloc(&b->krest, rest_bytes(b)), "checkpoint");
         BCHK(b, hipMalloc(&b->kcrushed, b->pitch), "checkpoint");
         BCHK(b, hipMalloc(&b->korig, b->pitch * sizeof(int)), "checkpoint");
+        if (b->grain) BCHK(b, hipMalloc(&b->kgrain, b->pitch * sizeof(short)), "checkpoint");
     }
     BCHK(b, hipMemcpyAsync(b->kpos, b->pos, pos_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
     BCHK(b, hipMemcpyAsync(b->krest, b->rest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
     BCHK(b, hipMemcpyAsync(b->kcrushed, b->crushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "checkpoint");
     BCHK(b, hipMemcpyAsync(b->korig, b->orig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+    if (b->grain)   // re-sorts permute the grain row with the points
+        BCHK(b, hipMemcpyAsync(b->kgrain, b->grain, b->pitch * sizeof(short), hipMemcpyDeviceToDevice, b->stream),
+             "checkpoint");
     BCHK(b, hipStreamSynchronize(b->stream), "checkpoint");
     b->kt = b->t;
     b->kstep = b->step;
@@ -1854,6 +2017,9 @@ int icempm_batch_restore(icempm_batch *b) {
     BCHK(b, hipMemcpyAsync(b->rest, b->krest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "restore");
     BCHK(b, hipMemcpyAsync(b->crushed, b->kcrushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "restore");
     BCHK(b, hipMemcpyAsync(b->orig, b->korig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "restore");
+    if (b->grain)
+        BCHK(b, hipMemcpyAsync(b->grain, b->kgrain, b->pitch * sizeof(short), hipMemcpyDeviceToDevice, b->stream),
+             "restore");
     BCHK(b, hipStreamSynchronize(b->stream), "restore");
     b->t = b->kt;
     b->step = b->kstep;
@@ -1867,23 +2033,39 @@ int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int 
     if (!b || probe_steps < 16 || cap < 0 || (cap > 0 && !report) || !rows) return ICEMPM_E_ARG;
     if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "autotune: batch not finalized");
     if (b->timingStride > 0) return bfail(b, ICEMPM_E_STATE, "autotune: switch kernel timing off first");
+    // the probes would append their pairs to the harvest buffer
+    if (b->rmap == kMapHarvest) return bfail(b, ICEMPM_E_STATE, "autotune: not in harvest mode");
     int rc = icempm_batch_checkpoint(b);
     if (rc) return rc;
+    // what the probes change besides the point state: the map statistics
+    // and the error flag (restored below), the force history and lastRun
+    // (the probes' own; a caller reads them after its next run)
+    unsigned long long stats0[3] = {0, 0, 0};
+    int err0 = 0;
+    BCHK(b, hipMemcpy(stats0, b->stats, sizeof(stats0), hipMemcpyDeviceToHost), "autotune");
+    BCHK(b, hipMemcpy(&err0, b->err, sizeof(int), hipMemcpyDeviceToHost), "autotune");
     const int chunks[3] = {256, 128, 64};
     const int resorts[3] = {0, probe_steps / 2, probe_steps / 4};
     double best = 1e300;
     int bestChunk = b->chunkPoints, bestResort = b->resortEvery, nrow = 0;
     for (int ci = 0; ci < 3; ++ci)
         for (int ri = 0; ri < 3; ++ri) {
-            // the run's time: device time of the graph launch without
-            // re-sorting; with re-sorting, the host time of every segment
-            // (re-sort, capture of the new chunk table, launch)
-            if ((rc = icempm_batch_restore(b))) return rc;
+            // one clock for every candidate: the host time of the whole
+            // probe (re-sorts and graph captures included), measured on a
+            // second run so that the no-re-sort candidates replay a graph
+            // captured by the first
             b->chunkPoints = chunks[ci];
             b->resortEvery = resorts[ri];
-            if ((rc = rebuild_chunks(b))) return rc;
-            if ((rc = icempm_batch_run(b, probe_steps, 1))) return rc;
-            const double per = b->lastMs / probe_steps;
+            double per = 0;
+            for (int pass = 0; pass < 2; ++pass) {
+                if ((rc = icempm_batch_restore(b))) return rc;
+                b->chunkPoints = chunks[ci];
+                if ((rc = rebuild_chunks(b))) return rc;
+                const auto h0 = std::chrono::steady_clock::now();
+                if ((rc = icempm_batch_run(b, probe_steps, 1))) return rc;
+                per = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count() /
+                      probe_steps;
+            }
             if (nrow < cap) {
                 report[3 * nrow] = chunks[ci];
                 report[3 * nrow + 1] = resorts[ri];
@@ -1899,6 +2081,8 @@ int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int 
     *rows = nrow;
     // back to the checkpointed state, with the fastest setting
     if ((rc = icempm_batch_restore(b))) return rc;
+    BCHK(b, hipMemcpy(b->stats, stats0, sizeof(stats0), hipMemcpyHostToDevice), "autotune");
+    BCHK(b, hipMemcpy(b->err, &err0, sizeof(int), hipMemcpyHostToDevice), "autotune");
     b->chunkPoints = bestChunk;
     b->resortEvery = bestResort;
     return rebuild_chunks(b);

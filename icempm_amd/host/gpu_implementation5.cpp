@@ -79,6 +79,10 @@ void GPU_Implementation5::cuda_transfer_from_device()
 {
     check(icempm_transfer_from_device(ctx, &GPU_Implementation5::callback_from_stream, this),
           "cuda_transfer_from_device");
+    // the reference's flag copy into pageable memory returns once it has
+    // landed (gpu_implementation5.cu:539-540): error_code is this cycle's
+    check(icempm_transfer_wait(ctx), "cuda_transfer_from_device");
+    error_code = icempm_error_code(ctx);
 }
 
 void GPU_Implementation5::callback_from_stream(void *userData)

@@ -129,8 +129,17 @@ typedef void (*icempm_callback)(void *user);
  * cb(user) on the runtime's callback thread when they have landed (cb may
  * be NULL).  The callback must not call into this library. */
 int icempm_transfer_from_device(icempm_ctx *ctx, icempm_callback cb, void *user);
-/* GPU_Implementation5::error_code: the host copy written by the last
- * completed icempm_transfer_from_device (lags like the reference). */
+/* Waits until the copies queued by the last icempm_transfer_from_device
+ * (points, bins, error flag) have landed in the pinned buffers -- not for
+ * its callback -- and refreshes error_code from the copied flag.  The
+ * reference's cuda_transfer_from_device copies the flag into the pageable
+ * member error_code (gpu_implementation5.cu:539-540), a copy that returns
+ * only once it has completed, so Model::Step (model.cpp:46) tests the flag
+ * of the cycle it has just run; GPU_Implementation5 calls this for the same
+ * effect.  No-op when nothing is pending. */
+int icempm_transfer_wait(icempm_ctx *ctx);
+/* GPU_Implementation5::error_code: the flag of the last readback whose
+ * copies have landed (icempm_transfer_wait or the readback callback). */
 int icempm_error_code(const icempm_ctx *ctx);
 /* Blocking read of the device error flag (no counterpart; for tests). */
 int icempm_read_error_flag(icempm_ctx *ctx, int *flag);
@@ -196,14 +205,18 @@ int icempm_read_bins(icempm_ctx *ctx, double *out);
  * one grid-update launch and one particle-pass launch over all cases, and
  * the substep loop can be captured in a HIP graph and replayed.  Cases are
  * added from the reference SoA layout (16 rows x pitch doubles, as read back
- * by icempm_read_points after LoadRawPoints); all cases share the time step;
- * the material of a case must be uniform (GrainVariability = 0).
+ * by icempm_read_points after LoadRawPoints); all cases share the time step.
+ * A case with GrainVariability != 0 takes each point's NACC strengths from
+ * its grain id (row 0 bytes [pitch, pitch + 2n), GetParametersForGrain,
+ * gpu_implementation5.cu:97-111); otherwise its material is uniform.
  * Precision: ICEMPM_BATCH_F64 stores the point state in FP64;
  * ICEMPM_BATCH_F32 stores velocity, Fe - I, Bp and Jp_inv - 1 in FP32 and the
- * positions as 32-bit fixed point over the case's grid width (a quantum of
- * GridX * cellsize / 2^32); the grid sums stay FP64, the grid velocities
- * reach the gathers as FP32 pairs, and all arithmetic is FP64.  The per-substep total
- * indenter force of every case is recorded; per-bin forces are not. */
+ * positions as 48-bit fixed point over the case's grid width (a quantum of
+ * GridX * cellsize / 2^48); the grid sums stay FP64, the grid velocities
+ * reach the gathers as FP32 pairs, and all arithmetic is FP64.  The
+ * per-substep total indenter force of every case is recorded, and its
+ * angular bins (Indenter_2D, gpu_implementation5.cu:262-269) summed over
+ * each icempm_batch_run call. */
 typedef struct icempm_batch icempm_batch;
 enum { ICEMPM_BATCH_F64 = 0, ICEMPM_BATCH_F32 = 1 };
 int icempm_batch_create(icempm_batch **out, int precision);
@@ -230,7 +243,12 @@ int icempm_batch_kernel_timing(icempm_batch *b, int stride);
 int icempm_batch_kernel_timing_read(icempm_batch *b, double *total_ms, int *launches);
 /* total indenter force (fx, fy) of case c for each substep of the last run */
 int icempm_batch_force_history(icempm_batch *b, int c, double *out, int n);
-/* the state of case c in the reference SoA layout (P, Q and grain zero) */
+/* indenter force bins of case c summed over the last icempm_batch_run call:
+ * out receives 2*nbins doubles (fx, fy interleaved per angular bin, the
+ * layout of host_side_indenter_force_accumulator); nbins must be the case's
+ * n_indenter_subdivisions */
+int icempm_batch_force_bins(icempm_batch *b, int c, double *out, int nbins);
+/* the state of case c in the reference SoA layout (P and Q zero) */
 int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch);
 int icempm_batch_error_flag(icempm_batch *b, int *flag);
 /* Tuning knobs: ICEMPM_BATCH_TUNE_CHUNK, points per chunk = threads per
@@ -249,10 +267,13 @@ int icempm_batch_resort(icempm_batch *b);
 int icempm_batch_checkpoint(icempm_batch *b);
 int icempm_batch_restore(icempm_batch *b);
 /* Autotuner: from a checkpoint of the current state, times probe_steps
- * substeps (graph replay) for every points-per-chunk value and the re-sort
- * cadences {never, probe/2, probe/4}; report receives rows (chunk, re-sort,
- * ms per substep), *rows their count; the state is restored and the fastest
- * setting is left in place. */
+ * substeps for every points-per-chunk value and the re-sort cadences
+ * {never, probe/2, probe/4}, each on the host clock over a second probe run
+ * (re-sorts and graph captures included; without re-sorting the second run
+ * replays the graph the first captured); report receives rows (chunk,
+ * re-sort, ms per substep), *rows their count.  The point state, the map
+ * statistics and the error flag are restored and the fastest setting is
+ * left in place.  Refused in harvest mode (the probes would append pairs). */
 int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int cap, int *rows);
 /* Return map of crushed points (gpu_implementation5.cu:48-94 is the exact
  * one).  ICEMPM_MAP_EXACT: the exact FP64 map.  ICEMPM_MAP_HARVEST: the

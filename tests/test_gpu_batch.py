@@ -31,10 +31,13 @@ def oracle_case(p0, soa0, nsub):
 
 
 # Positions: FP64 storage keeps them to 1e-12 m.  FP32 storage holds them as
-# 32-bit fixed point over the grid width (quantum gx h / 2^32, ~3e-10 m on
-# these grids); the rounding of each substep's store walks a coordinate by
-# some tens of quanta over 150 substeps, hence 2e-8 m (2e-6 of a cell).
-@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 2e-8)],
+# 48-bit fixed point over the grid width (quantum gx h / 2^48, ~1e-14 m on
+# these grids) and Fe - I in FP32 (~6e-8 of itself, which moves the stress
+# by as much): 1e-10 m over 150 substeps (1e-8 of a cell).
+F32_TOL_X = 1e-10
+
+
+@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, F32_TOL_X)],
                          ids=["f64", "f32"])
 def test_batch_matches_oracle_per_case(gpu_available, precision, tol_f, tol_x):
     print("batch", precision, flush=True)
@@ -116,7 +119,7 @@ def test_batch_state_round_trip(gpu_available, precision):
     """A finalized batch holds every case's points in band order and, for FP32
     storage, positions in fixed point: read back before any substep, each case
     returns its ingested state in the original point order — exactly for FP64
-    storage; for FP32 to within half a position quantum (gx h / 2^33) and the
+    storage; for FP32 to within half a position quantum (gx h / 2^49) and the
     FP32 rounding of the other rows."""
     print("batch round trip", precision, flush=True)
     cases = batch_cases(4, 700, 5000, seed=17)
@@ -130,8 +133,10 @@ def test_batch_state_round_trip(gpu_available, precision):
             for r in range(3, 16):   # Jp_inv, position, velocity, Fe, Bp
                 assert np.array_equal(dev[r, :n], soa0[r, :n]), r
         else:
-            q = p.GridX * p.cellsize / 2.0 ** 32
+            q = p.GridX * p.cellsize / 2.0 ** 48
             assert np.abs(dev[4:6, :n] - soa0[4:6, :n]).max() <= 0.5 * q * (1 + 1e-9)
+            # the ingested positions are floats: 48 bits hold most of them exactly
+            assert np.mean(dev[4:6, :n] == soa0[4:6, :n]) > 0.5
             for r in list(range(6, 16)) + [3]:
                 ref = soa0[r, :n].astype(np.float32).astype(np.float64)
                 if r in (8, 11, 3):   # Fe00, Fe11 and Jp_inv are stored less 1
@@ -140,7 +145,7 @@ def test_batch_state_round_trip(gpu_available, precision):
     b.close()
 
 
-@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 2e-8)],
+@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 1e-9)],
                          ids=["f64", "f32"])
 def test_mass_entering_empty_grid_blocks(gpu_available, precision, tol_f, tol_x):
     """Every point starts with an upward velocity of 50 m/s, so over 300
@@ -181,8 +186,8 @@ def test_grid_at_the_width_limit(gpu_available):
     states = ingest_cases([case])
     href, pref = oracle_case(*states[0], nsub)
     n = case.n_points
-    q = case.grid_x * case.h / 2.0 ** 32      # the FP32 position quantum on this grid, 9e-8 m
-    for precision, tol_f, tol_x in ((BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 40 * q)):
+    q = case.grid_x * case.h / 2.0 ** 48      # the FP32 position quantum on this grid, 1.4e-12 m
+    for precision, tol_f, tol_x in ((BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, max(40 * q, F32_TOL_X))):
         b = build_batch([case], precision, states)
         b.run(nsub)
         assert b.error_flag() == 0
@@ -220,9 +225,20 @@ def test_batch_call_order_and_argument_errors(gpu_available):
     other_dt.InitialTimeStep = p.InitialTimeStep * 2
     b.add_state(cases[0], p, soa)
     refused(ICEMPM_E_ARG, "time step", b.add_state, cases[1], other_dt, states[1][1])
-    grains = states[1][0].copy()
-    grains.GrainVariability = 0.5
-    refused(ICEMPM_E_ARG, "uniform material", b.add_state, cases[1], grains, states[1][1])
+    # sizes and derived constants the kernels index and scale by are checked
+    # on the host, before any kernel runs
+    bad = states[1][0].copy()
+    bad.GridTotal = bad.GridX * bad.GridY - 1
+    refused(ICEMPM_E_ARG, "GridTotal", b.add_state, cases[1], bad, states[1][1])
+    bad = states[1][0].copy()
+    bad.cellsize_inv = 1.01 / bad.cellsize
+    refused(ICEMPM_E_ARG, "consistent", b.add_state, cases[1], bad, states[1][1])
+    bad = states[1][0].copy()
+    bad.cellsize = -bad.cellsize
+    refused(ICEMPM_E_ARG, "positive", b.add_state, cases[1], bad, states[1][1])
+    bad = states[1][0].copy()
+    bad.n_indenter_subdivisions = 0
+    refused(ICEMPM_E_ARG, "n_indenter_subdivisions", b.add_state, cases[1], bad, states[1][1])
     b.add_state(cases[1], *states[1])
     b.finalize()
     refused(ICEMPM_E_STATE, "already finalized", b.finalize)
@@ -262,4 +278,101 @@ def test_tiny_and_ragged_cases(gpu_available, precision, tol_f, tol_x):
         href, pref = oracle_case(*b.initial[c], nsub)
         assert rel_l2(b.force_history(c, nsub), href) <= tol_f, c
         assert np.abs(b.points(c)[4:6, :n] - pref[4:6, :n]).max() <= tol_x, c
+    b.close()
+
+
+def grain_states(cases, gv=0.5, seed=5):
+    """Ingested states with per-point grain ids drawn from [-40, 1000) and
+    GrainVariability gv (the reference's default material); the last case
+    keeps a uniform material, so one batch mixes both."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, (p, soa) in enumerate(ingest_cases(cases)):
+        p, soa = p.copy(), soa.copy()
+        if i < len(cases) - 1:
+            n = p.nPts
+            pitch = soa.shape[1]
+            soa[0].view(np.uint8)[pitch:pitch + 2 * n].view(np.int16)[:] = rng.integers(-40, 1000, size=n)
+            p.GrainVariability = gv
+        out.append((p, soa))
+    return out
+
+
+@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, F32_TOL_X)],
+                         ids=["f64", "f32"])
+def test_batch_grain_variability_matches_oracle(gpu_available, precision, tol_f, tol_x):
+    """Cases with per-grain strengths (GetParametersForGrain,
+    gpu_implementation5.cu:97-111) next to a uniform one: each follows the
+    oracle's run of it, crushed flags included, and the grain ids read back
+    unchanged -- also after a re-sort and a checkpoint/restore, which move
+    the grain row with the points."""
+    print("batch: grain variability", precision, flush=True)
+    cases = batch_cases(3, 2000, 6000, seed=61)
+    states = grain_states(cases)
+    nsub = 200
+    refs = [oracle_case(*states[c], nsub) for c in range(len(cases))]
+    crushed = [int(r[1][0].view(np.uint8)[:cases[c].n_points].sum()) for c, r in enumerate(refs)]
+    print("oracle crushed points per case", crushed, flush=True)
+    assert min(crushed[:-1]) >= 5
+    b = build_batch(cases, precision, states)
+
+    def check(label):
+        assert b.error_flag() == 0
+        for c, case in enumerate(cases):
+            n = case.n_points
+            href, pref = refs[c]
+            dev = b.points(c)
+            e = rel_l2(b.force_history(c, nsub), href)
+            assert e <= tol_f, (label, c, e)
+            assert np.abs(dev[4:6, :n] - pref[4:6, :n]).max() <= tol_x, (label, c)
+            assert np.array_equal(dev[0].view(np.uint8)[:n], pref[0].view(np.uint8)[:n]), (label, c)
+            pitch = dev.shape[1]
+            g_dev = dev[0].view(np.uint8)[pitch:pitch + 2 * n].view(np.int16)
+            s0 = states[c][1]
+            g_ref = s0[0].view(np.uint8)[s0.shape[1]:s0.shape[1] + 2 * n].view(np.int16)
+            assert np.array_equal(g_dev, g_ref), (label, c)
+
+    b.checkpoint()
+    b.run(nsub)
+    check("straight")
+    b.restore()
+    b.set_tuning(resort_every=70)
+    b.run(nsub)
+    check("re-sorted every 70")
+    b.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(BATCH_F64, 1e-6), (BATCH_F32, 1e-3)], ids=["f64", "f32"])
+def test_batch_force_bins_match_oracle(gpu_available, precision, tol):
+    """Each case's angular indenter-force bins (Indenter_2D,
+    gpu_implementation5.cu:262-269), summed over a run, against the oracle's
+    bins over the same substeps; they also sum to the run's force history."""
+    print("batch: force bins", precision, flush=True)
+    cases = batch_cases(3, 2000, 6000, seed=71)
+    nsub = 300
+    b = build_batch(cases, precision)
+    b.run(nsub)
+    assert b.error_flag() == 0
+    for c, case in enumerate(cases):
+        p0, soa0 = b.initial[c]
+        p = SimParams()
+        ctypes.memmove(ctypes.byref(p), ctypes.byref(p0), ctypes.sizeof(SimParams))
+        assert nsub < p.UpdateEveryNthStep      # one cycle: the oracle's bins sum over all of it
+        pts = soa0.copy()
+        grid = oracle.new_grid(p)
+        bins = np.zeros(2 * p.n_indenter_subdivisions)
+        oracle.run(p, pts, grid, bins, nsub)
+        got = b.force_bins(c)
+        assert np.count_nonzero(bins) >= 4           # several bins in contact
+        assert np.array_equal(got != 0, bins != 0) or precision == BATCH_F32
+        e = rel_l2(got, bins)
+        assert e <= tol, (c, e)
+        tot = b.force_history(c, nsub).sum(axis=0)
+        assert np.allclose(got.reshape(-1, 2).sum(axis=0), tot, rtol=1e-9, atol=1e-9 * np.abs(tot).max())
+    # a second run starts its bins from zero
+    b.run(10)
+    for c in range(len(cases)):
+        tot = b.force_history(c, 10).sum(axis=0)
+        assert np.allclose(b.force_bins(c).reshape(-1, 2).sum(axis=0), tot, rtol=1e-9,
+                           atol=1e-9 * max(np.abs(tot).max(), 1e-30))
     b.close()

@@ -45,16 +45,16 @@ def single_case_fp64(case, nsub):
 def test_plain_at_scale_fp32_storage(gpu_available):
     """1.6M points, 300 substeps.  FP64 batch storage follows the single-case
     engine to the rounding of the sums' order; FP32 storage keeps the
-    indenter force within 1e-3 (relative L2) and the positions within 1e-6 m
-    (2e-4 of a cell; the position quantum on this 6 m grid is 1.4e-9 m, and
-    the rounding of each substep's stored state walks positions by about 130
-    quanta, 1.9e-7 m, over 300 substeps)."""
+    indenter force within 1e-3 (relative L2) and the positions within 1e-9 m
+    (2e-7 of a cell; the 48-bit position quantum on this 6 m grid is
+    2e-14 m, and the FP32 rounding of Fe - I moves the stress by ~6e-8 of
+    itself)."""
     case, nsub = INDENT_1M6, 300
     print("plain at scale", case.name, flush=True)
     p0, soa0, h64, x64 = single_case_fp64(case, nsub)
     n = case.n_points
     assert np.abs(h64).sum() > 0
-    for precision, tol_f, tol_x in ((BATCH_F64, 1e-9, 1e-12), (BATCH_F32, 1e-3, 1e-6)):
+    for precision, tol_f, tol_x in ((BATCH_F64, 1e-9, 1e-12), (BATCH_F32, 1e-3, 1e-9)):
         print("one-case batch", precision, flush=True)
         b = build_batch([case], precision, [(p0, soa0)])
         try:
@@ -77,15 +77,15 @@ def test_batched_64_cases_full_size(gpu_available):
     FP64 storage: the smallest case equals the oracle's run of it to 1e-6
     (relative L2 of the force history) over the whole horizon.
 
-    FP32 storage against FP64: almost every case crushes within its first
-    50 substeps, so the whole horizon is past fracture onset, where a
+    FP32 storage against FP64, every case within the FP32 bar of 1e-3
+    (relative L2 of the force history).  Almost every case crushes within its
+    first 50 substeps, so the whole horizon is past fracture onset, where a
     perturbation of a point's state changes the substep at which neighbours
     cross the failure surface.  Replacing one FP32 rounding at a time in an
-    FP64 run shows where the difference comes from: Fe - I and Jp_inv - 1 add
-    ~1e-6; the 32-bit position quantum (gx h / 2^32, 0.3-1.2 nm here) gives
-    the rest, median 2e-6, above 1e-3 in 2 of the 64 cases and 1.2% in the
-    worst.  The bars: every case within 2%, at least 60 of 64 within 1e-3,
-    the median within 1e-4."""
+    FP64 run showed where the difference came from: Fe - I and Jp_inv - 1
+    add ~1e-6; positions held on 32 bits (a quantum of gx h / 2^32,
+    0.3-1.2 nm here) put 2 of the 64 cases above 1e-3, the worst at 1.2%.
+    Positions are held on 48 bits now (a quantum 65,536 times finer)."""
     cases = batch_cases(64, 20000, 250000, 2024)
     nsub = 2000
     states = ingest_cases(cases)
@@ -117,8 +117,7 @@ def test_batched_64_cases_full_size(gpu_available):
     r = np.array([rel_l2(h32, h64) for h32, h64 in zip(hist[BATCH_F32], hist[BATCH_F64])])
     print(f"FP32 vs FP64: max {r.max():.3e} (case {r.argmax()}), median {np.median(r):.3e}, "
           f"above 1e-3: {(r > 1e-3).sum()}", flush=True)
-    assert r.max() <= 2e-2
-    assert (r <= 1e-3).sum() >= 60
+    assert r.max() <= 1e-3
     assert np.median(r) <= 1e-4
 
 

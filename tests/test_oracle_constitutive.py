@@ -108,16 +108,95 @@ def test_grain_params_uniform_material_matches_nacc_constants():
     assert math.sqrt(g["Msq"]) == pytest.approx(1.2, rel=1e-12)
 
 
-@pytest.mark.parametrize("grain", list(range(0, 30)) + [100, 999])
-def test_grain_variability_table(grain):
+def test_grain_msq_equals_camclay_msq_of_the_grain_strengths():
+    """For every grain id 0..999 at GrainVariability 0.5, GetParametersForGrain's
+    M^2 = 4 q^2 (1+2b) / (p (1+b))^2 (gpu_implementation5.cu:107) equals the
+    square of ComputeCamClayParams2's M = 2 q sqrt(1+2b) / (p (1+b))
+    (parameters_sim.cpp:141-146, run by the host library) evaluated on that
+    grain's (pmax, -pmin, qmax); and beta = -pmin / pmax."""
     p = params(GrainVariability=0.5)
-    g = oracle.grain_params(p, grain)
-    v1 = 1 + 0.5 * 0.05 * (-10 + grain % 21)
-    v2 = 1 + 0.5 * 0.033 * (-15 + (grain + 3) % 30)
-    v3 = 1 + 0.5 * 0.1 * (-10 + (grain + 4) % 11)
-    assert g["pmax"] == pytest.approx(p.IceCompressiveStrength * v1, rel=1e-14)
-    assert g["pmin"] == pytest.approx(-p.IceTensileStrength * v2, rel=1e-14)
-    assert g["qmax"] == pytest.approx(p.IceShearStrength * v3, rel=1e-14)
+    for grain in range(1000):
+        g = oracle.grain_params(p, grain)
+        q = params(IceCompressiveStrength=g["pmax"], IceTensileStrength=-g["pmin"],
+                   IceShearStrength=g["qmax"])
+        assert g["beta"] == pytest.approx(q.NACC_beta, rel=1e-15), grain
+        assert g["Msq"] == pytest.approx(q.NACC_Msq, rel=4e-15), grain
+
+
+def test_grain_factors_are_periodic_with_21_30_11():
+    """pmax repeats every 21 grain ids, pmin every 30, qmax every 11 (the
+    moduli of GetParametersForGrain), and no shorter period fits."""
+    p = params(GrainVariability=0.5)
+    rows = [oracle.grain_params(p, g) for g in range(0, 700)]
+    for key, period in (("pmax", 21), ("pmin", 30), ("qmax", 11)):
+        v = np.array([r[key] for r in rows])
+        assert np.array_equal(v[period:], v[:-period]), key
+        for shorter in range(1, period):
+            assert not np.array_equal(v[shorter:], v[:-shorter]), (key, shorter)
+
+
+@pytest.mark.parametrize("gv", [0.5, 0.37])
+def test_grain_factors_are_one_at_their_centres(gv):
+    """Each factor is exactly 1 where its term -10 + g%21, -15 + (g+3)%30,
+    -10 + (g+4)%11 vanishes: g = 10 (mod 21), 12 (mod 30), 6 (mod 11)."""
+    p = params(GrainVariability=gv)
+    for g in range(0, 1000):
+        r = oracle.grain_params(p, g)
+        if g % 21 == 10:
+            assert r["pmax"] == p.IceCompressiveStrength, g
+        else:
+            assert r["pmax"] != p.IceCompressiveStrength, g
+        if g % 30 == 12:
+            assert r["pmin"] == -p.IceTensileStrength, g
+        if g % 11 == 6:
+            assert r["qmax"] == p.IceShearStrength, g
+
+
+# Hand-computed factors at GrainVariability 0.5 (var = 1 + 0.5 * step * term):
+#   g = 0    terms -10, -12, -6   -> 0.75,   0.802,  0.70
+#   g = 37   37 % 21 = 16, 40 % 30 = 10, 41 % 11 = 8
+#            terms   6,  -5, -2   -> 1.15,   0.9175, 0.90
+#   g = -1   C++ % truncates: -1 % 21 = -1, 2 % 30 = 2, 3 % 11 = 3
+#            terms -11, -13, -7   -> 0.725,  0.7855, 0.65
+#            (a floored modulo would give 20, 2, 3 -> 1.25, 0.7855, 0.65)
+#   g = -25  -25 % 21 = -4, -22 % 30 = -22, -21 % 11 = -10
+#            terms -14, -37, -20  -> 0.65,   0.3895, 0.0
+HAND_FACTORS = {0: (0.75, 0.802, 0.70), 37: (1.15, 0.9175, 0.90), -1: (0.725, 0.7855, 0.65),
+                -25: (0.65, 0.3895, 0.0)}
+
+
+@pytest.mark.parametrize("grain", sorted(HAND_FACTORS))
+def test_grain_params_hand_values(grain):
+    p = params(GrainVariability=0.5)
+    r = oracle.grain_params(p, grain)
+    f1, f2, f3 = HAND_FACTORS[grain]
+    assert r["pmax"] == pytest.approx(p.IceCompressiveStrength * f1, rel=1e-14, abs=1e-6)
+    assert r["pmin"] == pytest.approx(-p.IceTensileStrength * f2, rel=1e-14, abs=1e-6)
+    assert r["qmax"] == pytest.approx(p.IceShearStrength * f3, rel=1e-14, abs=1e-6)
+
+
+@pytest.mark.parametrize("grain", [0, 37, 101, 999, -1])
+@pytest.mark.parametrize("frac", [0.1, 0.5, 0.9])
+def test_failure_check_uses_the_grain_surface(grain, frac):
+    """With GrainVariability 0.5 the NACC test follows the grain's own
+    ellipse and tension line: points just inside and just outside the
+    per-grain boundaries, computed from the grain's strengths by the
+    closed forms q_b = sqrt(-M^2 (p + b pmax)(p - pmax) / (1 + 2b)) and
+    q_b = q0 (p + 3e6) / 3e6 with q0 = 2 sqrt(-pmax pmin) qmax / (pmax - pmin)."""
+    p = params(GrainVariability=0.5)
+    r = oracle.grain_params(p, grain)
+    pmax, pmin, qmax = r["pmax"], r["pmin"], r["qmax"]
+    beta = -pmin / pmax
+    M2 = (2 * qmax * math.sqrt(1 + 2 * beta) / (pmax * (1 + beta))) ** 2
+    pp = frac * pmax
+    qb = math.sqrt(-M2 * (pp + beta * pmax) * (pp - pmax) / (1 + 2 * beta))
+    assert oracle.failure_check(p, grain, pp, qb * (1 - 1e-9)) == 0
+    assert oracle.failure_check(p, grain, pp, qb * (1 + 1e-9)) == 1
+    pt = -frac * 3e6
+    q0 = 2 * math.sqrt(-pmax * pmin) * qmax / (pmax - pmin)
+    qt = q0 * (pt + 3e6) / 3e6
+    assert oracle.failure_check(p, grain, pt, qt * (1 - 1e-9)) == 0
+    assert oracle.failure_check(p, grain, pt, qt * (1 + 1e-9) + 1e-9) == 1
 
 
 def ellipse_q(p, g, pp):
