@@ -89,7 +89,18 @@ def test_grain_ordered_bit_exact(gpu_available, pass_kind):
     say("grain ordered: crushed", int(cr.sum()), "grains of crushed", grains_of(pts, n)[cr == 1][:12])
     # the horizon decides real flags, on grains whose strengths differ from grain 0's
     assert cr.sum() >= 5
-    bad = np.flatnonzero((dev[:, :n] != pts[:, :n]).any(axis=0))
+    # compared as bits: row 0 packs crushed flags and grain ids, and four
+    # negative grain ids side by side read as a NaN double
+    du, pu = dev.view(np.uint64), pts.view(np.uint64)
+    bad = np.flatnonzero((du[:, :n] != pu[:, :n]).any(axis=0))
+    if bad.size:
+        rows = [r for r in range(1, 16) if (du[r, :n] != pu[r, :n]).any()]
+        hdiff = np.flatnonzero((hist_dev != hist_ref).any(axis=1))
+        say("grain ordered: rows that differ", rows,
+            {r: float(np.abs(dev[r, :n] - pts[r, :n]).max()) for r in rows},
+            "crushed flags differ at", np.flatnonzero(crushed_of(dev, n) != cr)[:10],
+            "first force-history difference at substep", hdiff[:3],
+            "grains of the first differing points", grains_of(pts, n)[bad[:10]])
     assert bad.size == 0, f"{bad.size} particles differ, first {bad[:5]}"
     assert np.array_equal(dev.view(np.uint64), pts.view(np.uint64))
     assert np.array_equal(hist_dev, hist_ref)
