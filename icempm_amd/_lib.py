@@ -124,7 +124,7 @@ HIP_EXPORTS = [
     "icempm_g2p", "icempm_run_substeps", "icempm_force_history", "icempm_transfer_from_device",
     "icempm_error_code", "icempm_transfer_wait", "icempm_read_error_flag", "icempm_event_record_start",
     "icempm_event_record_stop", "icempm_event_elapsed_ms", "icempm_sort_points",
-    "icempm_sort_raw_points", "icempm_set_mode", "icempm_get_mode", "icempm_kernel_timing",
+    "icempm_sort_raw_points", "icempm_sort_raw_points_noalloc", "icempm_set_mode", "icempm_get_mode", "icempm_kernel_timing",
     "icempm_kernel_timing_read", "icempm_read_grid", "icempm_read_points",
     "icempm_write_grid", "icempm_read_bins", "icempm_set_tuning", "icempm_get_tuning",
     "icempm_measure_copy", "icempm_batch_create", "icempm_batch_destroy",
@@ -195,6 +195,8 @@ def hip():
     _sig(L, "icempm_event_elapsed_ms", c_int, P, POINTER(c_float))
     _sig(L, "icempm_sort_points", c_int, P, POINTER(c_int), POINTER(c_int))
     _sig(L, "icempm_sort_raw_points", c_int, P, POINTER(c_float), POINTER(c_int16), c_int,
+         c_double, c_int)
+    _sig(L, "icempm_sort_raw_points_noalloc", c_int, P, POINTER(c_float), POINTER(c_int16), c_int,
          c_double, c_int)
     _sig(L, "icempm_set_mode", c_int, P, c_int)
     _sig(L, "icempm_get_mode", c_int, P)

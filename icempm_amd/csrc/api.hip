@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -391,6 +392,48 @@ struct ReadbackJob {
     void *user;
 };
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free sort (icempm_sort_raw_points_noalloc): one
+// static device array, a copy on every GPU the library runs on, held until
+// the process ends.  In host code `scratch` names a host-side stand-in, not
+// a device address: icempm_create looks the array up on its device with
+// hipGetSymbolAddress and keeps that address per device, and only that
+// address reaches kernels and copies.  Layout for n <= kNoAllocMaxPoints, in
+// 256-byte aligned pieces: staged xy in and out (8 B a point each), grains in
+// and out (2 B), keys, values and their ping-pong copies (4 B each), radix
+// histograms and scan sums.
+constexpr size_t kNoAllocMaxPoints = ICEMPM_NOALLOC_MAX_POINTS;
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+// radix_tmp_ints(n) = 2 h + scan_tmp_ints(h) + 64 with h = 16 ceil(n / 2048);
+// scan_tmp_ints(h) < h / 2048 + 64 per level: 3 h + 1024 ints bound it
+constexpr size_t kRadixTmpInts = 3 * (16 * ((kNoAllocMaxPoints + 2047) / 2048)) + 1024;
+constexpr size_t kScratchBytes = 2 * al256(kNoAllocMaxPoints * 8) + 2 * al256(kNoAllocMaxPoints * 2) +
+                                 4 * al256(kNoAllocMaxPoints * 4) + al256(kRadixTmpInts * 4);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+unsigned char *g_scratchPtr[kMaxDevices] = {};
+
+// the device address of `scratch` on `dev` (current), looked up once per device
+hipError_t scratch_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratchPtr[dev]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratchPtr[dev] = static_cast<unsigned char *>(ptr);
+    return hipSuccess;
+}
+
+unsigned char *scratch_of(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    return g_scratchPtr[dev];
+}
+
 void readback_done(void *p) {
     ReadbackJob *j = static_cast<ReadbackJob *>(p);
     j->ctx->errorCode.store(*j->ctx->hostErr);
@@ -419,6 +462,7 @@ int icempm_create(icempm_ctx **out) {
     icempm_ctx *c = new icempm_ctx();
     c->device = 0;
     e = hipSetDevice(0);
+    if (e == hipSuccess) e = scratch_setup(c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->evStart);
     if (e == hipSuccess) e = hipEventCreate(&c->evStop);
@@ -795,6 +839,54 @@ int icempm_sort_raw_points(icempm_ctx *c, float *xy, int16_t *grains, int n, dou
     (void)hipStreamSynchronize(c->stream);
     dfree(dxy); dfree(dxy2); dfree(dg); dfree(dg2);
     if (e != hipSuccess) return hipfail(c, e, "sort_raw_points");
+    return ICEMPM_OK;
+}
+
+int icempm_sort_raw_points_noalloc(icempm_ctx *c, float *xy, int16_t *grains, int n, double hinv,
+                                   int gridX) {
+    if (!c || n < 0 || (n > 0 && (!xy || !grains))) return ICEMPM_E_ARG;
+    if ((size_t)n > kNoAllocMaxPoints)
+        return fail(c, ICEMPM_E_ARG, "sort_raw_points_noalloc: n = " + std::to_string(n) +
+                                         " exceeds ICEMPM_NOALLOC_MAX_POINTS = " + std::to_string(kNoAllocMaxPoints));
+    if (n <= 1) return ICEMPM_OK;
+    if (radix_tmp_ints((size_t)n) > kRadixTmpInts)
+        return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: radix scratch bound too small");
+    unsigned char *base = scratch_of(c->device);
+    if (!base) return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: scratch not set up on this device");
+    unsigned char *q = base;
+    auto take = [&q](size_t bytes) {
+        unsigned char *r = q;
+        q += al256(bytes);
+        return r;
+    };
+    float *dxy = reinterpret_cast<float *>(take(kNoAllocMaxPoints * 8));
+    float *dxy2 = reinterpret_cast<float *>(take(kNoAllocMaxPoints * 8));
+    int16_t *dg = reinterpret_cast<int16_t *>(take(kNoAllocMaxPoints * 2));
+    int16_t *dg2 = reinterpret_cast<int16_t *>(take(kNoAllocMaxPoints * 2));
+    int *keys = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
+    int *vals = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
+    int *keys2 = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
+    int *vals2 = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
+    int *tmp = reinterpret_cast<int *>(take(kRadixTmpInts * 4));
+    if ((size_t)(q - base) > kScratchBytes)
+        return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: scratch layout exceeds the array");
+    // every piece is written before it is read (keys and values by k_raw_keys,
+    // histograms and scan sums by their kernels each pass): nothing to clear
+    hipError_t e = hipMemcpyAsync(dxy, xy, 2 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dg, grains, (size_t)n * sizeof(int16_t), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_raw_keys(dxy, n, hinv, gridX, keys, vals, c->stream);
+    if (e == hipSuccess) e = radix_sort_pairs(keys, vals, keys2, vals2, (size_t)n, tmp, c->stream);
+    if (e == hipSuccess) e = launch_permute_raw(dxy, dg, dxy2, dg2, n, vals, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(xy, dxy2, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(grains, dg2, (size_t)n * sizeof(int16_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(c->stream);
+        return hipfail(c, e, "sort_raw_points_noalloc");
+    }
     return ICEMPM_OK;
 }
 

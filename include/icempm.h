@@ -166,6 +166,21 @@ int icempm_sort_points(icempm_ctx *ctx, int *perm_out, int *keys_out);
 int icempm_sort_raw_points(icempm_ctx *ctx, float *xy, int16_t *grains, int n, double hinv,
                            int gridX);
 
+/* -- allocation-free variant (no counterpart in the reference) ----------
+ * For callers that cannot allocate device memory (serving loops, harnesses
+ * that forbid hipMalloc): icempm_sort_raw_points with the same arguments,
+ * results and blocking behaviour, but every device buffer it needs -- the
+ * staged points and grains, the keys, the values and the radix scratch --
+ * lives in a static device array of the library (about 36 MB, one copy on
+ * each GPU the library runs on, held until the process ends) whose address
+ * icempm_create looks up.  It calls no hipMalloc or hipFree.  n may be at
+ * most ICEMPM_NOALLOC_MAX_POINTS (ICEMPM_E_ARG with a message otherwise).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several threads or contexts) race. */
+#define ICEMPM_NOALLOC_MAX_POINTS 1048576
+int icempm_sort_raw_points_noalloc(icempm_ctx *ctx, float *xy, int16_t *grains, int n, double hinv,
+                                   int gridX);
+
 /* -- configuration (no counterpart) ------------------------------------- */
 int icempm_set_mode(icempm_ctx *ctx, int mode);
 int icempm_get_mode(const icempm_ctx *ctx);
