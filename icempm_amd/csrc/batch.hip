@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/icempm.h"
@@ -75,6 +76,17 @@ struct Rows {
 
 constexpr double kFix48Max = 281474976710655.0;   // 2^48 - 1
 
+// Element i of a row through a 32-bit byte offset from the row's start: the
+// row start is wavefront-uniform (scalar registers), so the access takes the
+// scalar-base + 32-bit vector-offset form, with no 64-bit address arithmetic
+// per lane.  Rows hold < 2^29 elements (finalize refuses larger batches), so
+// i * sizeof(T) < 2^32.
+template <class T>
+static __device__ __forceinline__ T *at(T *row, unsigned i) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T *>(reinterpret_cast<B *>(row) + i * (unsigned)sizeof(T));
+}
+
 // FP32 storage: a coordinate as 48-bit fixed point, u = hi 2^16 + lo
 // (round to nearest; outside [0, gx h) and NaN go to the grid's edge, where
 // the next stencil check flags them).  Every step is exact in FP64 except
@@ -100,26 +112,28 @@ static __host__ __device__ inline bool fix48_in_range(double x, const CaseParams
 template <bool NT>
 static __device__ __forceinline__ void pos_load(const double *pos, size_t pp, int pt, const CaseParams &,
                                                 double &x, double &y) {
+    const unsigned u = (unsigned)pt;
     if constexpr (NT) {
-        x = __builtin_nontemporal_load(&pos[pt]);
-        y = __builtin_nontemporal_load(&pos[pp + pt]);
+        x = __builtin_nontemporal_load(at(pos, u));
+        y = __builtin_nontemporal_load(at(pos + pp, u));
     } else {
-        x = pos[pt];
-        y = pos[pp + pt];
+        x = *at(pos, u);
+        y = *at(pos + pp, u);
     }
 }
 template <bool NT>
 static __device__ __forceinline__ void pos_load(const unsigned int *pos, size_t pp, int pt, const CaseParams &cp,
                                                 double &x, double &y) {
+    const unsigned u = (unsigned)pt;
     unsigned int hx, hy, lo;
     if constexpr (NT) {
-        hx = __builtin_nontemporal_load(&pos[pt]);
-        hy = __builtin_nontemporal_load(&pos[pp + pt]);
-        lo = __builtin_nontemporal_load(&pos[2 * pp + pt]);
+        hx = __builtin_nontemporal_load(at(pos, u));
+        hy = __builtin_nontemporal_load(at(pos + pp, u));
+        lo = __builtin_nontemporal_load(at(pos + 2 * pp, u));
     } else {
-        hx = pos[pt];
-        hy = pos[pp + pt];
-        lo = pos[2 * pp + pt];
+        hx = *at(pos, u);
+        hy = *at(pos + pp, u);
+        lo = *at(pos + 2 * pp, u);
     }
     x = fix48_get(hx, lo & 0xFFFFu, cp);
     y = fix48_get(hy, lo >> 16, cp);
@@ -127,17 +141,19 @@ static __device__ __forceinline__ void pos_load(const unsigned int *pos, size_t 
 // stores (x, y) and replaces them by the values the rows now hold
 static __device__ __forceinline__ void pos_store(double *pos, size_t pp, int pt, const CaseParams &, double &x,
                                                  double &y) {
-    __builtin_nontemporal_store(x, &pos[pt]);
-    __builtin_nontemporal_store(y, &pos[pp + pt]);
+    const unsigned u = (unsigned)pt;
+    __builtin_nontemporal_store(x, at(pos, u));
+    __builtin_nontemporal_store(y, at(pos + pp, u));
 }
 static __device__ __forceinline__ void pos_store(unsigned int *pos, size_t pp, int pt, const CaseParams &cp,
                                                  double &x, double &y) {
+    const unsigned u = (unsigned)pt;
     unsigned int hx, lx, hy, ly;
     fix48_put(x, cp, hx, lx);
     fix48_put(y, cp, hy, ly);
-    __builtin_nontemporal_store(hx, &pos[pt]);
-    __builtin_nontemporal_store(hy, &pos[pp + pt]);
-    __builtin_nontemporal_store(lx | (ly << 16), &pos[2 * pp + pt]);
+    __builtin_nontemporal_store(hx, at(pos, u));
+    __builtin_nontemporal_store(hy, at(pos + pp, u));
+    __builtin_nontemporal_store(lx | (ly << 16), at(pos + 2 * pp, u));
     x = fix48_get(hx, lx, cp);
     y = fix48_get(hy, ly, cp);
 }
@@ -155,7 +171,7 @@ static __device__ __forceinline__ void grid_v(const double *g, int node, size_t 
     v1 = g[2 * gpitch + node];
 }
 static __device__ __forceinline__ void grid_v(const float2 *g, int node, size_t, float &v0, float &v1) {
-    const float2 a = g[node];
+    const float2 a = *at(g, (unsigned)node);
     v0 = a.x;
     v1 = a.y;
 }
@@ -194,13 +210,6 @@ static __device__ __forceinline__ KParams kp_of(const CaseParams &c, int *err) {
     k.ics = c.ics; k.its = c.its; k.iss = c.iss; k.tanPhi = c.tanPhi; k.dpThr = c.dpThr;
     k.grainVar = c.grainVar;
     return k;
-}
-
-// a[k] <- a[(k + s) % 3] for s in {0, 1}
-template <class T>
-static __device__ __forceinline__ void rot3(bool s, T &a0, T &a1, T &a2) {
-    const T t0 = s ? a1 : a0, t1 = s ? a2 : a1, t2 = s ? a0 : a2;
-    a0 = t0; a1 = t1; a2 = t2;
 }
 
 // Return map of crushed points: the exact Drucker-Prager map, the exact map
@@ -326,7 +335,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
     double F00 = 1, F01 = 0, F10 = 0, F11 = 1;
     double B00 = 0, B01 = 0, B10 = 0, B11 = 0;
     bool defer = false;   // learned map: projection and P2G done by the map kernel
-    Inv rt{};             // FP64 storage: the trial invariants, and whether F is still their F
+    InvSq rt{};           // the trial invariants, and whether the map left F alone
     bool fresh = false;
 
     if constexpr (G2P) {
@@ -335,20 +344,21 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         short grain = 0;
         if (valid) {
             // streamed once per substep: non-temporal, so the grid keeps the L2
+            const unsigned u = (unsigned)pt;
             pos_load<true>(S.pos, pp, pt, cp, px, py);
-            F00 = fe0 + (double)__builtin_nontemporal_load(&R[pp * 2 + pt]);
-            F01 = (double)__builtin_nontemporal_load(&R[pp * 3 + pt]);
-            F10 = (double)__builtin_nontemporal_load(&R[pp * 4 + pt]);
-            F11 = fe0 + (double)__builtin_nontemporal_load(&R[pp * 5 + pt]);
-            crushed = __builtin_nontemporal_load(&S.crushed[pt]);
-            if (cp.grainVar != 0) grain = __builtin_nontemporal_load(&grains[pt]);
+            F00 = fe0 + (double)__builtin_nontemporal_load(at(R + pp * 2, u));
+            F01 = (double)__builtin_nontemporal_load(at(R + pp * 3, u));
+            F10 = (double)__builtin_nontemporal_load(at(R + pp * 4, u));
+            F11 = fe0 + (double)__builtin_nontemporal_load(at(R + pp * 5, u));
+            crushed = __builtin_nontemporal_load(at(S.crushed, u));
+            if (cp.grainVar != 0) grain = __builtin_nontemporal_load(at(grains, u));
         }
         int bx = 0, by = 0;
         double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
         const bool ok = valid && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
                         stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
         if (valid && !ok) flag_error(k);
-        Inv iv{};
+        InvSq iv{};
         unsigned char crushed0 = crushed;
         double Jp0 = Jp;
         if (ok) {
@@ -418,17 +428,21 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             const double n00 = M00 * F00 + M01 * F10, n01 = M00 * F01 + M01 * F11;
             const double n10 = M10 * F00 + M11 * F10, n11 = M10 * F01 + M11 * F11;
             F00 = n00; F01 = n01; F10 = n10; F11 = n11;
-            iv = invariants_fast(F00, F01, F10, F11, cp.mu, cp.kappa);
+            // p and q^2 of the trial state (q itself is never needed here:
+            // every test compares it with a non-negative limit)
+            iv = invariants_sq(F00, F01, F10, F11, cp.mu, cp.kappa);
             crushed0 = crushed;
             Jp0 = Jp;
             if (crushed == 0) {
                 bool f;
                 if (cp.grainVar != 0) {   // GetParametersForGrain's strengths (case-uniform branch)
-                    f = fails(k, grain, iv.p, iv.q);
+                    f = fails_sq(k, grain, iv.p, iv.q2);
                 } else if (iv.p < 0) {
-                    f = (iv.p < -3e6) || (iv.q > cp.slope * (iv.p - -3e6));
+                    // tension line q > slope (p + 3e6), slope >= 0 and p + 3e6 >= 0 here
+                    const double ql = cp.slope * (iv.p - -3e6);
+                    f = (iv.p < -3e6) || (iv.q2 > ql * ql);
                 } else {
-                    f = cp.onePlus2Beta * iv.q * iv.q + cp.msq * (iv.p + cp.betaPmax) * (iv.p - cp.pmax) > 0;
+                    f = cp.onePlus2Beta * iv.q2 + cp.msq * (iv.p + cp.betaPmax) * (iv.p - cp.pmax) > 0;
                 }
                 crushed = f ? 1 : 0;
             }
@@ -441,11 +455,11 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         // Jp_inv matters only to the map, so only crushed points load it.
         const bool crushedOk = ok && crushed == 1;
         if (crushedOk) {
-            Jp = fe0 + (double)R[pp * 10 + pt];
+            Jp = fe0 + (double)*at(R + pp * 10, (unsigned)pt);
             Jp0 = Jp;
         }
         if constexpr (RMAP == kMapExact) {
-            if (crushedOk && dp_projects(k, iv.p, iv.q, Jp))
+            if (crushedOk && dp_projects_sq(k, iv.p, iv.q2, Jp))
                 dp_return(k, compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa), F00, F01, F10, F11, Jp);
         } else if constexpr (RMAP == kMapHarvest) {
             Trial r{};
@@ -519,10 +533,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 if (ins) atomicAdd(&la.stats[2], (unsigned long long)__popcll(ins));
             }
         }
-        if constexpr (!F32) {
-            rt = iv;
-            fresh = ok && crushed == 0;
-        }
+        // a point the map left alone keeps the stress of its trial state: the
+        // P2G takes it from the invariants (with FP32 storage they describe
+        // F before its rounding to the stored FP32, which differs from it by
+        // ~6e-8 of Fe - I)
+        rt = iv;
+        fresh = ok && crushed == 0;
         if (ok) {
             if constexpr (F32) {
                 // fixed point clamps to [0, gx h): a position outside (or
@@ -532,11 +548,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             // px, py become what the rows hold: the P2G takes them at their
             // stored precision
             pos_store(S.pos, pp, pt, cp, px, py);
+            const unsigned u = (unsigned)pt;
             if (!defer) {
-                __builtin_nontemporal_store((V)(F00 - fe0), &R[pp * 2 + pt]);
-                __builtin_nontemporal_store((V)F01, &R[pp * 3 + pt]);
-                __builtin_nontemporal_store((V)F10, &R[pp * 4 + pt]);
-                __builtin_nontemporal_store((V)(F11 - fe0), &R[pp * 5 + pt]);
+                __builtin_nontemporal_store((V)(F00 - fe0), at(R + pp * 2, u));
+                __builtin_nontemporal_store((V)F01, at(R + pp * 3, u));
+                __builtin_nontemporal_store((V)F10, at(R + pp * 4, u));
+                __builtin_nontemporal_store((V)(F11 - fe0), at(R + pp * 5, u));
             }
             // v and Bp: the fused pass uses them at once (a deferred point's
             // go to the map kernel in its queue entry) and the next G2P
@@ -550,8 +567,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 R[pp * 8 + pt] = (V)B10;
                 R[pp * 9 + pt] = (V)B11;
             }
-            if (Jp != Jp0 && !defer) R[pp * 10 + pt] = (V)(Jp - fe0);
-            if (crushed != crushed0) S.crushed[pt] = crushed;
+            if (Jp != Jp0 && !defer) *at(R + pp * 10, u) = (V)(Jp - fe0);
+            if (crushed != crushed0) *at(S.crushed, u) = crushed;
             if constexpr (F32) {
                 // Fe is what the next substep reads back: the P2G takes it at
                 // its stored precision.  v and Bp stay FP64: a fused pass
@@ -593,13 +610,14 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         // at a time
         const bool inb = ok && bx < gx && by < gy;
         const bool any = __ballot(inb) != 0;
-        const unsigned cell = (unsigned)bx | ((unsigned)by << 16);
-        const unsigned lo = wreduce_u16x2<false>(inb ? cell : 0xFFFFFFFFu);
-        const unsigned hi = wreduce_u16x2<true>(inb ? cell : 0u);
-        const int ox = any ? (int)(lo & 0xFFFFu) : INT_MAX;
-        const int oy = any ? (int)(lo >> 16) : INT_MAX;
-        const int bw = any ? min((int)(hi & 0xFFFFu) + 3 - ox, kWX) : 0;
-        const int bh = any ? min((int)(hi >> 16) + 3 - oy, kWY) : 0;
+        const unsigned lx = wave_min_u32(inb ? (unsigned)bx : 0xFFFFFFFFu);
+        const unsigned ly = wave_min_u32(inb ? (unsigned)by : 0xFFFFFFFFu);
+        const unsigned hx = wave_max_u32(inb ? (unsigned)bx : 0u);
+        const unsigned hy = wave_max_u32(inb ? (unsigned)by : 0u);
+        const int ox = any ? (int)lx : INT_MAX;
+        const int oy = any ? (int)ly : INT_MAX;
+        const int bw = any ? min((int)hx + 3 - ox, kWX) : 0;
+        const int bh = any ? min((int)hy + 3 - oy, kWY) : 0;
         const int bsh = bw <= 16 ? 4 : (bw <= 32 ? 5 : 6);
         const int bcol = lane & ((1 << bsh) - 1), brow0 = lane >> bsh, bstep = 64 >> bsh;
         double *W0 = smem[wv];
@@ -618,7 +636,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             if (fresh)
                 kirchhoff_from_inv(rt, t00, t01, t11);
             else
-                kirchhoff(F00, F01, F10, F11, cp.mu, cp.kappa, t00, t01, t11);
+                kirchhoff_rcp(F00, F01, F10, F11, cp.mu, cp.kappa, t00, t01, t11);
             // Every point of a case has the same mass m: the window sums the
             // contributions per unit mass, W (v + (Bp - m^-1 dt V D^-1 tau) d),
             // and the flush multiplies by m once per node.
@@ -630,27 +648,27 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             if (bx + 2 < gx && by + 2 < gy && (unsigned)lx0 + 2 < (unsigned)kWX &&
                 (unsigned)ly0 + 2 < (unsigned)kWY) {
                 const int e0 = ly0 * kWS + lx0;
-                double xw0 = wx0, xw1 = wx1, xw2 = wx2;
-                double xa0 = vx + A00 * ((0 - fx0) * h), xa1 = vx + A00 * ((1 - fx0) * h),
-                       xa2 = vx + A00 * ((2 - fx0) * h);
-                double xb0 = vy + A10 * ((0 - fx0) * h), xb1 = vy + A10 * ((1 - fx0) * h),
-                       xb2 = vy + A10 * ((2 - fx0) * h);
-                double yw0 = wy0, yw1 = wy1, yw2 = wy2;
-                double ya0 = A01 * ((0 - fx1) * h), ya1 = A01 * ((1 - fx1) * h), ya2 = A01 * ((2 - fx1) * h);
-                double yb0 = A11 * ((0 - fx1) * h), yb1 = A11 * ((1 - fx1) * h), yb2 = A11 * ((2 - fx1) * h);
-                int xe0 = 0, xe1 = 1, xe2 = 2, ye0 = 0, ye1 = kWS, ye2 = 2 * kWS;
-                const bool sx = (lane & 1) != 0, sy = (lane & 2) != 0;
-                rot3(sx, xw0, xw1, xw2);
-                rot3(sx, xa0, xa1, xa2);
-                rot3(sx, xb0, xb1, xb2);
-                rot3(sx, xe0, xe1, xe2);
-                rot3(sy, yw0, yw1, yw2);
-                rot3(sy, ya0, ya1, ya2);
-                rot3(sy, yb0, yb1, yb2);
-                rot3(sy, ye0, ye1, ye2);
-                const double xw[3] = {xw0, xw1, xw2}, xa[3] = {xa0, xa1, xa2}, xb[3] = {xb0, xb1, xb2};
-                const double yw[3] = {yw0, yw1, yw2}, ya[3] = {ya0, ya1, ya2}, yb[3] = {yb0, yb1, yb2};
-                const int xe[3] = {xe0, xe1, xe2}, ye[3] = {ye0, ye1, ye2};
+                // The four lanes of a quad (in band order mostly the points of
+                // one cell) walk the 3 x 3 stencil from four different nodes,
+                // so their LDS atomics never meet at one address: lane bit 0
+                // shifts the columns, bit 1 the rows, by one.  Step k of an
+                // axis visits node (k + s) mod 3 = s, 1 + s, 2 - 2 s; only the
+                // weights need selecting, the offsets follow from the index.
+                const int sx = lane & 1, sy = (lane >> 1) & 1;
+                const int xe[3] = {sx, 1 + sx, 2 - 2 * sx};
+                const int iy[3] = {sy, 1 + sy, 2 - 2 * sy};
+                const int ye[3] = {iy[0] * kWS, iy[1] * kWS, iy[2] * kWS};
+                const double xw[3] = {sx ? wx1 : wx0, sx ? wx2 : wx1, sx ? wx0 : wx2};
+                const double yw[3] = {sy ? wy1 : wy0, sy ? wy2 : wy1, sy ? wy0 : wy2};
+                double xa[3], xb[3], ya[3], yb[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double dx = ((double)xe[q] - fx0) * h, dy = ((double)iy[q] - fx1) * h;
+                    xa[q] = vx + A00 * dx;
+                    xb[q] = vy + A10 * dx;
+                    ya[q] = A01 * dy;
+                    yb[q] = A11 * dy;
+                }
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -1561,8 +1579,9 @@ int icempm_batch_finalize(icempm_batch *b) {
         pofs += h.cp.nPts;
         gofs += h.cp.gridTotal;
     }
-    if (pofs > INT_MAX / 2 || gofs > INT_MAX / 2 || bofs > INT_MAX / 2)
-        return bfail(b, ICEMPM_E_ARG, "finalize: batch too large");
+    // the particle pass addresses rows through 32-bit byte offsets
+    if (pofs >= (1L << 29) || gofs >= (1L << 29) || bofs > INT_MAX / 2)
+        return bfail(b, ICEMPM_E_ARG, "finalize: batch too large (2^29 points or grid nodes at most)");
     b->binsTotal = (int)bofs;
     b->nPts = (int)pofs;
     b->nNodes = gofs;

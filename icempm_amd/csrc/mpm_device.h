@@ -261,12 +261,75 @@ static __device__ __forceinline__ void kirchhoff_from_inv(const Inv &v, double &
     t11 = -a - 2. * v.p;
 }
 
+// 1/x for a normal x: the hardware reciprocal (v_rcp_f64) refined by two
+// Newton-Raphson steps, r <- r + r (1 - x r), to within an ulp or two of the
+// correctly rounded quotient.  FAST mode only (not the IEEE division).
+static __device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// The FAST trial invariants with q^2 in place of q: every test on q in the
+// fused passes compares it with a non-negative limit (the NACC ellipse uses
+// q^2 itself), so no square root is taken, and c = mu / J comes from rcp_nr.
+//   q^2 = (mu / J)^2 ((b00 - b11)^2 + 4 b01^2)
+struct InvSq {
+    double c, d, b01, p, q2;   // c = mu / J, d = b00 - b11
+};
+
+static __device__ __forceinline__ InvSq invariants_sq(double F00, double F01, double F10, double F11,
+                                                      double mu, double kappa) {
+    InvSq v;
+    const double J = F00 * F11 - F10 * F01;
+    const double b00 = F00 * F00 + F01 * F01, b11 = F10 * F10 + F11 * F11;
+    v.b01 = F00 * F10 + F01 * F11;
+    v.d = b00 - b11;
+    v.c = mu * rcp_nr(J);
+    v.p = -(kappa / 2.) * (J * J - 1.);
+    v.q2 = (v.c * v.c) * (v.d * v.d + 4. * (v.b01 * v.b01));
+    return v;
+}
+
+static __device__ __forceinline__ void kirchhoff_from_inv(const InvSq &v, double &t00, double &t01,
+                                                          double &t11) {
+    const double a = 0.5 * (v.c * v.d);
+    t00 = a - 2. * v.p;
+    t01 = v.c * v.b01;
+    t11 = -a - 2. * v.p;
+}
+
+// KirchhoffStress_Wolper (gpu_implementation5.cu:24-34) with 1/J from rcp_nr
+static __device__ __forceinline__ void kirchhoff_rcp(double F00, double F01, double F10, double F11, double mu,
+                                                     double kappa, double &t00, double &t01, double &t11) {
+    const double Je = F00 * F11 - F10 * F01;
+    const double b00 = F00 * F00 + F01 * F01;
+    const double b01 = F00 * F10 + F01 * F11;
+    const double b11 = F10 * F10 + F11 * F11;
+    const double half_tr = (b00 + b11) / 2;
+    const double c1 = mu * rcp_nr(Je);
+    const double c2 = kappa * (Je * Je - 1.);
+    t00 = c1 * (b00 - half_tr) + c2;
+    t01 = c1 * b01;
+    t11 = c1 * (b11 - half_tr) + c2;
+}
+
 // Whether the Drucker-Prager map (dp_return) changes a crushed trial state
 // with these p and q: the tension branch, or q at or above the cone.
 static __device__ __forceinline__ bool dp_projects(const KParams &k, double p, double q, double Jp_inv) {
     if (p < -k.dpThr || Jp_inv < 1) return true;
     const double q_n_1 = (p > k.ics) ? 0.0 : fmin((p + k.dpThr) * k.tanPhi, k.iss);
     return q >= q_n_1;
+}
+
+// the same from q^2: past the tension test p + p_thr >= 0, so the cone's
+// q_n_1 is >= 0 (for tan(phi), qmax >= 0) and q >= q_n_1 iff q^2 >= q_n_1^2
+static __device__ __forceinline__ bool dp_projects_sq(const KParams &k, double p, double q2, double Jp_inv) {
+    if (p < -k.dpThr || Jp_inv < 1) return true;
+    const double q_n_1 = (p > k.ics) ? 0.0 : fmin((p + k.dpThr) * k.tanPhi, k.iss);
+    return q_n_1 <= 0.0 || q2 >= q_n_1 * q_n_1;
 }
 
 // NACC test with the constants of grain 0 (GrainVariability == 0)
@@ -319,6 +382,31 @@ static __device__ __forceinline__ bool fails(const KParams &k, short grain, doub
         return q > q_limit;
     }
     double y = (1. + 2. * beta) * q * q + mSq * (p + beta * pmax) * (p - pmax);
+    return y > 0;
+}
+
+// fails() from q^2: the tension line's q_limit is >= 0 for qmax >= 0, and
+// q > q_limit iff q^2 > q_limit^2 there; a negative q_limit (qmax < 0, a
+// GrainVariability above 1) is exceeded by every q >= 0
+static __device__ __forceinline__ bool fails_sq(const KParams &k, short grain, double p, double q2) {
+    double var1 = 1.0 + k.grainVar * 0.05 * (-10 + grain % 21);
+    double var2 = 1.0 + k.grainVar * 0.033 * (-15 + (grain + 3) % 30);
+    double var3 = 1.0 + k.grainVar * 0.1 * (-10 + (grain + 4) % 11);
+    double pmax = k.ics * var1;
+    double pmin = -k.its * var2;
+    double qmax = k.iss * var3;
+    double beta = -pmin / pmax;
+    double a = pmax * (1 + beta);
+    double mSq = (4 * qmax * qmax * (1 + 2 * beta)) / (a * a);
+    const double pmin2 = -3e6;
+    if (p < 0) {
+        if (p < pmin2) return true;
+        double q0 = 2 * sqrt(-pmax * pmin) * qmax / (pmax - pmin);
+        double kk = -q0 / pmin2;
+        double q_limit = kk * (p - pmin2);
+        return q_limit < 0 || q2 > q_limit * q_limit;
+    }
+    double y = (1. + 2. * beta) * q2 + mSq * (p + beta * pmax) * (p - pmax);
     return y > 0;
 }
 
@@ -401,6 +489,25 @@ static __device__ __forceinline__ int wreduce_i32(int v) {
     return op(op(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
               op(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
+
+// Wavefront minimum of an unsigned value, in lane 63's register read back
+// as a scalar: four DPP steps within each row of 16 lanes (quad_perm 1032,
+// 2301, half-row mirror, row mirror), then row_bcast:15 into rows 1 and 3
+// and row_bcast:31 into rows 2 and 3 (GFX9 DPP).  With the identity
+// (0xFFFFFFFF) as the DPP "old" value every step folds into one
+// v_min_u32_dpp.  Every lane of the wavefront must be active.
+static __device__ __forceinline__ unsigned wave_min_u32(unsigned v) {
+    const int I = (int)0xFFFFFFFFu;
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(I, (int)v, 0xB1, 0xF, 0xF, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(I, (int)v, 0x4E, 0xF, 0xF, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(I, (int)v, 0x141, 0xF, 0xF, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(I, (int)v, 0x140, 0xF, 0xF, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(I, (int)v, 0x142, 0xA, 0xF, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(I, (int)v, 0x143, 0xC, 0xF, false));
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+// maximum, as the complement of the minimum of complements
+static __device__ __forceinline__ unsigned wave_max_u32(unsigned v) { return ~wave_min_u32(~v); }
 
 // The same for the two 16-bit halves of a word at once.
 typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));

@@ -148,8 +148,11 @@ def test_batch_state_round_trip(gpu_available, precision):
 
 # FP32 storage: the gathers read each node's velocity as an FP32 pair, a
 # relative error of ~6e-8 that displaces a point by as much of its travel
-# (0.03 m here): 2e-8 m covers it.
-@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 2e-8)],
+# (0.03 m here), ~2e-9 m for most points.  The flight crushes ~90% of the
+# points, and a point's FP32-stored plastic state can take it ~5e-7 m off
+# the oracle's path (measured; which point depends on the atomic order).
+# The skipped-block logic under test would be off by a cell if it failed.
+@pytest.mark.parametrize("precision,tol_f,tol_x", [(BATCH_F64, 1e-6, 1e-12), (BATCH_F32, 1e-3, 1e-6)],
                          ids=["f64", "f32"])
 def test_mass_entering_empty_grid_blocks(gpu_available, precision, tol_f, tol_x):
     """Every point starts with an upward velocity of 50 m/s, so over 300
