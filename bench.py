@@ -16,9 +16,11 @@ and FP32 storage and the exact map.
 5 m x 2 m, 1,600,000 points, grid 1200 x 500).
 
 Prints one JSON line: particle-updates/s, the roofline of the dominant
-kernel (the fused G2P+P2G pass, timed with HIP events on its stream), and the
-CPU baseline (the C++ restatement, OpenMP, on this host's cores, on a
-bounded sample of the same workload).
+kernel (the fused G2P+P2G pass, timed with HIP events on its stream; the HBM
+traffic per launch from the committed rocprofv3 counter summary), and the
+CPU baseline (the C++ restatement's timing build, OpenMP, on this host's
+cores at 1 thread and at 16, on a bounded sample of every case of the same
+workload).
 """
 from __future__ import annotations
 
@@ -95,7 +97,27 @@ def atomics_from_profiles(name: str, avg_launch_s: float):
     return {"l2_atomic_requests_per_launch": n, "l2_atomic_requests_per_s": n / avg_launch_s}
 
 
-def cpu_baseline(case, pts0, prms0, target_s: float, threads: int):
+def cpu_info() -> dict:
+    """nproc and the CPU model of this host (the GPU box's host when run there)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline(states, target_s: float, label: str) -> dict:
+    """The oracle's timing build (oracle/liboracle_fast.so: g++ -O3
+    -march=x86-64-v3 -fopenmp -ffp-contract=off, bit-identical to the parity
+    build) on this host's cores, over every case of the workload: one
+    warm-up substep of each case, then as many substeps of each case as fit
+    in about target_s / 3 seconds at 1 thread and target_s * 2/3 at all
+    threads.  All threads = the 16 host cores that go with one GPU on the
+    box (or fewer if the host has fewer)."""
     import ctypes
 
     import numpy as np
@@ -103,26 +125,44 @@ def cpu_baseline(case, pts0, prms0, target_s: float, threads: int):
     import oracle
     from icempm_amd import SimParams
 
-    p = SimParams()
-    ctypes.memmove(ctypes.byref(p), ctypes.byref(prms0), ctypes.sizeof(SimParams))
-    pts = np.ascontiguousarray(pts0.copy())
-    grid = oracle.new_grid(p)
-    bins = np.zeros(2 * p.n_indenter_subdivisions)
-    oracle.run(p, pts, grid, bins, 1, nthreads=threads)          # warm-up
-    t0 = time.perf_counter()
-    oracle.run(p, pts, grid, bins, 1, nthreads=threads)
-    one = time.perf_counter() - t0
-    nsub = max(1, min(20000, int(target_s / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    _, err = oracle.run(p, pts, grid, bins, nsub, nthreads=threads)
-    el = time.perf_counter() - t0
+    info = cpu_info()
+    threads_all = max(1, min(info["nproc"] or 1, 16))
+
+    def copy_state(p0, soa):
+        p = SimParams()
+        ctypes.memmove(ctypes.byref(p), ctypes.byref(p0), ctypes.sizeof(SimParams))
+        return p, np.ascontiguousarray(soa.copy())
+
+    def rate(threads, seconds):
+        work = [copy_state(p0, soa) for p0, soa in states]
+        grids = [oracle.new_grid(p) for p, _ in work]
+        nts = sum(p.nPts for p, _ in work)
+        t0 = time.perf_counter()
+        for (p, pts), g in zip(work, grids):       # warm-up: one substep of each case
+            oracle.run(p, pts, g, np.zeros(2 * p.n_indenter_subdivisions), 1, nthreads=threads, fast=True)
+        one = time.perf_counter() - t0
+        nsub = max(1, int(seconds / max(one, 1e-9)))
+        t0 = time.perf_counter()
+        for (p, pts), g in zip(work, grids):
+            oracle.run(p, pts, g, np.zeros(2 * p.n_indenter_subdivisions), nsub, nthreads=threads, fast=True)
+        el = time.perf_counter() - t0
+        return nts * nsub / el, nsub, el
+
+    v1, n1, e1 = rate(1, target_s / 3)
+    va, na, ea = rate(threads_all, target_s * 2 / 3)
     return {
-        "value": case.n_points * nsub / el,
+        "value": va,
         "unit": "particle-updates/s",
-        "cores": threads,
+        "cores": threads_all,
         "kind": "port",
-        "sample": f"{case.name}: {nsub} substeps after 2 warm-up substeps, OpenMP {threads} threads, "
-                  f"{el:.1f} s",
+        "sample": f"{label}: every case, 1 warm-up substep then {na} substeps each at {threads_all} threads "
+                  f"({ea:.1f} s) and {n1} substeps each at 1 thread ({e1:.1f} s); oracle timing build "
+                  f"(g++ -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off)",
+        "threads_1": v1,
+        "threads_all": va,
+        "threads_all_count": threads_all,
+        "nproc": info["nproc"],
+        "cpu_model": info["cpu_model"],
     }
 
 
@@ -185,9 +225,8 @@ def run_single(args) -> dict:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        log(f"bench: CPU baseline ({threads} threads, ~{args.cpu_seconds:.0f} s)")
-        cpu = cpu_baseline(case, pts0, prms0, args.cpu_seconds, threads)
+        log(f"bench: CPU baseline (~{args.cpu_seconds:.0f} s)")
+        cpu = cpu_baseline([(prms0, pts0)], args.cpu_seconds, case.name)
 
     return {
         "metric": METRIC,
@@ -259,7 +298,9 @@ def run_batch_learned(args) -> dict:
         f"{time.perf_counter() - t0:.1f} s; warmup={warmup} steps={steps}")
 
     def timed(b, label):
-        """warm-up, then the timed steps (graph replay); the force histories of both runs"""
+        """W untimed warm-up substeps, then the K timed ones (one graph launch;
+        the graph is captured, instantiated and uploaded before its launch is
+        timed); the force histories of both"""
         b.run(warmup)
         h0 = [b.force_history(c, warmup) for c in range(len(cases))]
         ms = b.run(steps)
@@ -273,13 +314,19 @@ def run_batch_learned(args) -> dict:
         log(f"bench: {label}: {ms / steps * 1e3:.1f} us/substep, {pups:.3e} particle-updates/s")
         return pups, ms, hist
 
-    # 1. exact FP64 batched run: force reference and harvested pairs
+    # 1. the exact FP64 batched run from the initial state: the force
+    #    reference of the measured run (its first W + K substeps) and the
+    #    training pairs, harvested over a fixed horizon whatever --steps is
+    horizon = max(args.harvest_steps, warmup + steps)
     b = build_batch(cases, BATCH_F64, states)
     b.set_harvest(args.harvest)
-    f64_harvest_pups, _, ref_hist = timed(b, "batched FP64 exact (harvest)")
+    b.run(horizon)
+    if b.error_flag():
+        raise RuntimeError("bench: harvest run: out-of-grid point")
+    ref_hist = [b.force_history(c, warmup + steps) for c in range(len(cases))]
     X, T, offered = b.read_harvest(args.harvest)
     b.close()
-    log(f"bench: harvested {offered} pairs, training on {X.shape[0]}")
+    log(f"bench: exact FP64 run over {horizon} substeps: harvested {offered} pairs, training on {X.shape[0]}")
 
     # 2. the MLP, trained on the GPU
     t0 = time.perf_counter()
@@ -288,17 +335,24 @@ def run_batch_learned(args) -> dict:
     log(f"bench: trained {args.train_steps} Adam steps in {train_s:.1f} s, loss {losses[-1][1]:.3e}")
 
     # 3. the measured configuration: batched FP32 + learned map, autotuned
-    #    (points per chunk and re-sort cadence, timed from a checkpoint)
+    #    (points per chunk and re-sort cadence, probed from a checkpoint of
+    #    the initial state, which the tuner restores)
     b = build_batch(cases, BATCH_F32, states)
     b.set_learned(params, norm, tol=args.tol)
     tune_rows = []
     if args.autotune_steps > 0:
         t0 = time.perf_counter()
         tune_rows = b.autotune(args.autotune_steps)
-        b.map_stats(reset=True)
         log(f"bench: autotune {time.perf_counter() - t0:.1f} s -> chunk {b.tuning()[0]}, "
             f"re-sort every {b.tuning()[1]}: " + ", ".join(f"({c},{r}) {t * 1e3:.0f} us" for c, r, t in tune_rows))
     chunk, resort_every = b.tuning()
+    branches = b.map_branches()
+    probe_branch_stats = b.autotune_branch_stats() if args.autotune_steps > 0 else None
+    if probe_branch_stats:
+        ta, tf, sa, sf = probe_branch_stats
+        log(f"bench: network proposals passing the gate over the probes: tension {ta}/{ta + tf}, shear "
+            f"{sa}/{sa + sf} -> network on for branches {branches} (1 tension, 2 shear)")
+    b.map_stats(reset=True)
     value, ms, hist = timed(b, "batched FP32 learned")
     acc, fb, ins = b.map_stats(reset=True)
     errs = [rel_l2(h, r) for h, r in zip(hist, ref_hist)]
@@ -326,20 +380,23 @@ def run_batch_learned(args) -> dict:
     b.close()
     errs32 = [rel_l2(h, r) for h, r in zip(f32_hist, ref_hist)]
 
-    # 5. the plain MPM: the single-case FP64 engine, case by case
+    # 5. the plain MPM: the single-case FP64 engine, case by case, the same
+    #    substep counts (at least 100 timed substeps per case, so that a short
+    #    --steps still times more than launch overhead)
+    plain_steps = max(steps, args.plain_min_steps)
     seq_updates, seq_time = 0, 0.0
     m = Model(MODE_FAST)
     copy_gbs = None
     try:
         for c in cases:
             m.load_case(c)
-            m.run(min(warmup, 20))
+            m.run(warmup)
             m.synchronize()
             t0 = time.perf_counter()
-            m.run(args.plain_steps)
+            m.run(plain_steps)
             m.synchronize()
             seq_time += time.perf_counter() - t0
-            seq_updates += c.n_points * args.plain_steps
+            seq_updates += c.n_points * plain_steps
         copy_gbs = m.copy_bandwidth(1 << 30, 20)
     finally:
         m.close()
@@ -349,11 +406,10 @@ def run_batch_learned(args) -> dict:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        mid = sorted(range(len(cases)), key=lambda i: cases[i].n_points)[len(cases) // 2]
-        log(f"bench: CPU baseline ({threads} threads, ~{args.cpu_seconds:.0f} s, case {mid})")
-        cpu = cpu_baseline(cases[mid], states[mid][1], states[mid][0], args.cpu_seconds, threads)
+        log(f"bench: CPU baseline (~{args.cpu_seconds:.0f} s)")
+        cpu = cpu_baseline(states, args.cpu_seconds, f"the {len(cases)} cases of this workload")
 
+    traffic = traffic_from_profiles("batch_particles_learned_32cases")
     return {
         "metric": METRIC,
         "value": value,
@@ -377,7 +433,7 @@ def run_batch_learned(args) -> dict:
             "cases": len(cases),
             "points": n_total,
             "grid_nodes": nodes,
-            "storage": "f32 (positions 32-bit fixed point over each grid width)",
+            "storage": "f32 (positions 48-bit fixed point over each grid width)",
             "mlp": "bf16 weights/activations, f32 accumulation",
             "points_per_chunk": chunk,
             "resort_every": resort_every,
@@ -390,7 +446,8 @@ def run_batch_learned(args) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": traffic_from_profiles("batch_particles_learned_32cases"),
+            "traffic": traffic,
+            "hbm_frac_counters": (traffic / k_avg_s / 1e9 / HBM_PEAK_GBS) if (traffic and k_avg_s) else None,
             "atomics": atomics_from_profiles("batch_particles_learned_32cases", k_avg_s),
             "bytes_per_launch": k_bytes,
             "moved_bytes_per_launch": k_moved,
@@ -404,9 +461,8 @@ def run_batch_learned(args) -> dict:
         "substep_effective_gbs": k_bytes * steps / (ms / 1e3) / 1e9,
         "comparison": {
             "plain_single_case_f64_pups": plain_pups,
-            "plain_steps_per_case": args.plain_steps,
+            "plain_steps_per_case": plain_steps,
             "batched_f64_exact_pups": f64_pups,
-            "batched_f64_harvest_pups": f64_harvest_pups,
             "batched_f32_exact_pups": f32_pups,
             "batched_f32_learned_pups": value,
             "speedup_vs_plain": value / plain_pups,
@@ -417,6 +473,11 @@ def run_batch_learned(args) -> dict:
             "learned_fallback": fb,
             "crushed_inside_surface": ins,
             "fallback_rate": fb / max(acc + fb, 1),
+            "map_branches_network_on": branches,
+            "autotune_probe_branch_stats": (dict(zip(("tension_accepted", "tension_fell_back", "shear_accepted",
+                                                      "shear_fell_back"), probe_branch_stats))
+                                            if probe_branch_stats else None),
+            "harvest_substeps": horizon,
             "pairs_harvested": offered,
             "pairs_trained": int(X.shape[0]),
             "train_steps": args.train_steps,
@@ -442,10 +503,12 @@ def main() -> int:
     ap.add_argument("--n-max", type=int, default=500_000)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--tol", type=float, default=1e-4, help="batch: yield-residual tolerance")
-    ap.add_argument("--harvest", type=int, default=2_000_000, help="batch: pairs kept for training")
+    ap.add_argument("--harvest", type=int, default=4_000_000, help="batch: pairs kept for training")
+    ap.add_argument("--harvest-steps", type=int, default=3000,
+                    help="batch: substeps of the exact FP64 run that harvests the training pairs")
     ap.add_argument("--train-steps", type=int, default=3000)
-    ap.add_argument("--plain-steps", type=int, default=400,
-                    help="batch: substeps per case of the plain single-case comparison")
+    ap.add_argument("--plain-min-steps", type=int, default=100,
+                    help="batch: fewest timed substeps per case of the plain single-case comparison")
     ap.add_argument("--autotune-steps", type=int, default=200,
                     help="batch: substeps per autotuner candidate (0: keep the defaults)")
     ap.add_argument("--kernel-steps", type=int, default=200,

@@ -32,6 +32,9 @@ MAP_HARVEST = 1
 MAP_LEARNED = 2
 BATCH_TUNE_CHUNK = 0
 BATCH_TUNE_RESORT = 1
+BATCH_TUNE_MAP_MIN_ACCEPT = 2
+MAP_TENSION = 1
+MAP_SHEAR = 2
 
 
 class IcempmError(RuntimeError):
@@ -132,6 +135,8 @@ HIP_EXPORTS = [
     "icempm_batch_run", "icempm_batch_last_ms", "icempm_batch_num_points",
     "icempm_batch_force_history", "icempm_batch_force_bins", "icempm_batch_read_points", "icempm_batch_error_flag",
     "icempm_batch_set_return_map", "icempm_batch_read_harvest", "icempm_batch_map_stats",
+    "icempm_batch_map_branch_stats", "icempm_batch_set_map_branches", "icempm_batch_get_map_branches",
+    "icempm_batch_autotune_branch_stats",
     "icempm_batch_num_nodes", "icempm_batch_kernel_timing", "icempm_batch_kernel_timing_read",
     "icempm_batch_set_tuning", "icempm_batch_get_tuning", "icempm_batch_resort",
     "icempm_batch_checkpoint", "icempm_batch_restore", "icempm_batch_autotune",
@@ -226,6 +231,10 @@ def hip():
     _sig(L, "icempm_batch_read_harvest", c_int, P, POINTER(c_float), POINTER(c_float), c_int,
          POINTER(c_uint64))
     _sig(L, "icempm_batch_map_stats", c_int, P, POINTER(c_uint64), c_int)
+    _sig(L, "icempm_batch_map_branch_stats", c_int, P, POINTER(c_uint64), c_int)
+    _sig(L, "icempm_batch_set_map_branches", c_int, P, c_int)
+    _sig(L, "icempm_batch_get_map_branches", c_int, P)
+    _sig(L, "icempm_batch_autotune_branch_stats", c_int, P, POINTER(c_uint64))
     _sig(L, "icempm_batch_num_nodes", c_long, P)
     _sig(L, "icempm_batch_kernel_timing", c_int, P, c_int)
     _sig(L, "icempm_batch_kernel_timing_read", c_int, P, POINTER(c_double), POINTER(c_int))

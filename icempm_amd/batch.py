@@ -12,8 +12,8 @@ from ctypes import POINTER, byref, c_double, c_float, c_int, c_uint64, c_void_p
 import numpy as np
 
 from . import _lib
-from ._lib import (BATCH_F32, BATCH_F64, BATCH_TUNE_CHUNK, BATCH_TUNE_RESORT, MAP_EXACT, MAP_HARVEST,
-                   MAP_LEARNED, IcempmError, SimParams)
+from ._lib import (BATCH_F32, BATCH_F64, BATCH_TUNE_CHUNK, BATCH_TUNE_MAP_MIN_ACCEPT, BATCH_TUNE_RESORT,
+                   MAP_EXACT, MAP_HARVEST, MAP_LEARNED, MAP_SHEAR, MAP_TENSION, IcempmError, SimParams)
 from .model import MODE_FAST, IndentationCase, Model, nacc_strengths
 
 
@@ -201,6 +201,30 @@ class Batch:
                                                       int(cap), byref(tot)))
         n = min(int(tot.value), cap, getattr(self, "_hcap", 0))
         return X[:n], T[:n], int(tot.value)
+
+    def set_map_branches(self, mask: int) -> None:
+        """Branches the network proposes for (MAP_TENSION | MAP_SHEAR); the
+        others take the exact map in the particle pass."""
+        self._check(self._H.icempm_batch_set_map_branches(self._b, int(mask)))
+
+    def map_branches(self) -> int:
+        return int(self._H.icempm_batch_get_map_branches(self._b))
+
+    def set_map_min_accept(self, percent: int) -> None:
+        """Acceptance threshold below which autotune switches a branch off."""
+        self._check(self._H.icempm_batch_set_tuning(self._b, BATCH_TUNE_MAP_MIN_ACCEPT, int(percent)))
+
+    def map_branch_stats(self, reset: bool = False) -> tuple[int, int, int, int]:
+        """(tension accepted, tension fell back, shear accepted, shear fell back)"""
+        out = (c_uint64 * 4)()
+        self._check(self._H.icempm_batch_map_branch_stats(self._b, out, 1 if reset else 0))
+        return tuple(int(x) for x in out)
+
+    def autotune_branch_stats(self) -> tuple[int, int, int, int]:
+        """The last autotune's per-branch counts over its probes (as map_branch_stats)."""
+        out = (c_uint64 * 4)()
+        self._check(self._H.icempm_batch_autotune_branch_stats(self._b, out))
+        return tuple(int(x) for x in out)
 
     def map_stats(self, reset: bool = False) -> tuple[int, int, int]:
         """(points accepted from the MLP, points that fell back to the exact

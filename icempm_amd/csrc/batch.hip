@@ -225,8 +225,14 @@ struct LearnArgs {
     float *hx, *ht;              // harvest: inputs (n x 5), targets (n x 3)
     unsigned long long *hcount;
     int hcap;
-    unsigned long long *stats;   // [accepted, fell back, inside the surface]
+    // [accepted, fell back, inside the surface, tension accepted, tension fell
+    // back, shear accepted, shear fell back] (the last four: map kernel only)
+    unsigned long long *stats;
+    int branches;                // kMapTension | kMapShear: branches the network proposes for
 };
+
+constexpr int kMapStats = 7;
+enum { kMapTension = 1, kMapShear = 2 };
 
 // A crushed point whose projection is deferred to the map kernel: its trial
 // elastic deformation (FP64) and Jp_inv, and where it lives.
@@ -501,7 +507,11 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             if (crushedOk) r = compute_pq_fast(F00, F01, F10, F11, cp.mu, cp.kappa);
             const int br = crushedOk ? map_branch(k, r, Jp, qlim) : kBrInside;
             const bool proj = crushedOk && br != kBrInside;
-            defer = proj && r.svd.s1 > 0;
+            // the network proposes only in the branches it is enabled for
+            // (the autotuner switches off a branch whose proposals the yield
+            // gate does not pass); the others take the exact map here
+            const int bit = (br == kBrTension) ? kMapTension : kMapShear;
+            defer = proj && r.svd.s1 > 0 && (la.branches & bit) != 0;
             if (proj && !defer) dp_return(k, r, F00, F01, F10, F11, Jp);
             const unsigned long long dm = __ballot(defer);
             if (dm) {
@@ -852,9 +862,14 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
         }
         if (have && !accepted) dp_return(k, r, F00, F01, F10, F11, Jp);
         const unsigned long long acc = __ballot(accepted), fb = __ballot(have && !accepted);
+        const unsigned long long ten = __ballot(have && br == kBrTension);
         if (lane == 0) {
             if (acc) atomicAdd(&la.stats[0], (unsigned long long)__popcll(acc));
             if (fb) atomicAdd(&la.stats[1], (unsigned long long)__popcll(fb));
+            if (acc & ten) atomicAdd(&la.stats[3], (unsigned long long)__popcll(acc & ten));
+            if (fb & ten) atomicAdd(&la.stats[4], (unsigned long long)__popcll(fb & ten));
+            if (acc & ~ten) atomicAdd(&la.stats[5], (unsigned long long)__popcll(acc & ~ten));
+            if (fb & ~ten) atomicAdd(&la.stats[6], (unsigned long long)__popcll(fb & ~ten));
         }
         if (have) {
             const int pt = e.pt;
@@ -1084,7 +1099,10 @@ struct icempm_batch {
     float *hx = nullptr, *ht = nullptr;
     unsigned long long *hcount = nullptr;
     int hcap = 0;
-    unsigned long long *stats = nullptr;
+    unsigned long long *stats = nullptr;   // kMapStats counters
+    int mapBranches = kMapTension | kMapShear;   // branches the network proposes for
+    double mapMinAccept = 0.5;         // autotune: a branch accepted less often is switched off
+    unsigned long long tuneBranchStats[4] = {};   // the last autotune's per-branch accepted / fell back
     MapEntry *mapQ = nullptr;          // deferred points (learned map), nPts entries
     unsigned int *mapCount = nullptr;  // two counters, used alternately
 
@@ -1161,7 +1179,7 @@ void launch_one(icempm_batch *b, unsigned nb, const void *gV, double *gO, const 
                        static_cast<const typename Rows<F32>::G *>(gV), gO, b->gpitch,
                        b->err, b->grain, la, mq);
     if (b->evAfter) (void)hipEventRecord(b->evAfter, b->stream);
-    if constexpr (G2P && RMAP == kMapLearned) {
+    if (G2P && RMAP == kMapLearned && la.branches != 0) {
         // the deferred points: map, store, and their P2G into the pass's output grid
         hipLaunchKernelGGL((k_batch_map<F32>), dim3(kMapBlocks), dim3(256), 0, b->stream, S, b->cases, mq.q,
                            mq.count, mq.count == b->mapCount ? b->mapCount + 1 : b->mapCount, gO, b->gpitch,
@@ -1207,6 +1225,7 @@ hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const void *gV, doub
     la.hcount = b->hcount;
     la.hcap = b->hcap;
     la.stats = b->stats;
+    la.branches = b->mapBranches;
     MapQueue mq{b->mapQ, b->mapCount ? b->mapCount + (parity & 1) : nullptr, b->nPts};
     if (b->precision == ICEMPM_BATCH_F32)
         launch_prec<true>(b, nb, g2p, p2g, gV, gO, la, mq);
@@ -1605,7 +1624,7 @@ int icempm_batch_finalize(icempm_batch *b) {
     BCHK(b, hipMalloc(&b->blkFlags, 3 * nblk.size() * sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->err, sizeof(int)), "batch finalize");
     BCHK(b, hipMalloc(&b->hcount, sizeof(unsigned long long)), "batch finalize");
-    BCHK(b, hipMalloc(&b->stats, 3 * sizeof(unsigned long long)), "batch finalize");
+    BCHK(b, hipMalloc(&b->stats, kMapStats * sizeof(unsigned long long)), "batch finalize");
     // host staging in the device layout
     std::vector<double> hpos(2 * b->pitch, 0.0);
     std::vector<unsigned char> hrest(11 * b->pitch * vs, 0), hcr(b->pitch, 0);
@@ -1667,9 +1686,9 @@ int icempm_batch_finalize(icempm_batch *b) {
     BCHK(b, hipMemcpy(b->nodeBlock, nblk.data(), nblk.size() * sizeof(int2), hipMemcpyHostToDevice), "batch upload");
     int zero = 0;
     BCHK(b, hipMemcpy(b->err, &zero, sizeof(int), hipMemcpyHostToDevice), "batch upload");
-    const unsigned long long z3[3] = {0, 0, 0};
-    BCHK(b, hipMemcpy(b->hcount, z3, sizeof(unsigned long long), hipMemcpyHostToDevice), "batch upload");
-    BCHK(b, hipMemcpy(b->stats, z3, sizeof(z3), hipMemcpyHostToDevice), "batch upload");
+    const unsigned long long z7[kMapStats] = {};
+    BCHK(b, hipMemcpy(b->hcount, z7, sizeof(unsigned long long), hipMemcpyHostToDevice), "batch upload");
+    BCHK(b, hipMemcpy(b->stats, z7, sizeof(z7), hipMemcpyHostToDevice), "batch upload");
     int rc = upload_chunks(b);
     if (rc) return rc;
     b->finalized = true;
@@ -1700,6 +1719,10 @@ int run_segment(icempm_batch *b, int n, int use_graph, const double *tTab, doubl
             b->graphSteps = n;
             b->graphT = tTab;
             b->graphH = hist;
+            // put the executable graph on the device now, so that its first
+            // launch (timed below) costs what a replay costs
+            BCHK(b, hipGraphUpload(b->graph, b->stream), "batch graph upload");
+            BCHK(b, hipStreamSynchronize(b->stream), "batch graph upload");
         }
         // timed from here: capture and instantiation are host work before it
         BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
@@ -1980,6 +2003,10 @@ int icempm_batch_set_tuning(icempm_batch *b, int knob, int value) {
         if (value < 0) return bfail(b, ICEMPM_E_ARG, "set_tuning: the re-sort cadence is >= 0");
         b->resortEvery = value;
         return ICEMPM_OK;
+    case ICEMPM_BATCH_TUNE_MAP_MIN_ACCEPT:
+        if (value < 0 || value > 100) return bfail(b, ICEMPM_E_ARG, "set_tuning: acceptance threshold is 0..100 percent");
+        b->mapMinAccept = value / 100.0;
+        return ICEMPM_OK;
     default:
         return bfail(b, ICEMPM_E_ARG, "set_tuning: unknown knob");
     }
@@ -1990,6 +2017,7 @@ int icempm_batch_get_tuning(const icempm_batch *b, int knob) {
     switch (knob) {
     case ICEMPM_BATCH_TUNE_CHUNK: return b->chunkPoints;
     case ICEMPM_BATCH_TUNE_RESORT: return b->resortEvery;
+    case ICEMPM_BATCH_TUNE_MAP_MIN_ACCEPT: return (int)std::lround(b->mapMinAccept * 100.0);
     default: return -1;
     }
 }
@@ -2059,7 +2087,7 @@ int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int 
     // what the probes change besides the point state: the map statistics
     // and the error flag (restored below), the force history and lastRun
     // (the probes' own; a caller reads them after its next run)
-    unsigned long long stats0[3] = {0, 0, 0};
+    unsigned long long stats0[kMapStats] = {};
     int err0 = 0;
     BCHK(b, hipMemcpy(stats0, b->stats, sizeof(stats0), hipMemcpyDeviceToHost), "autotune");
     BCHK(b, hipMemcpy(&err0, b->err, sizeof(int), hipMemcpyDeviceToHost), "autotune");
@@ -2098,6 +2126,24 @@ int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int 
             }
         }
     *rows = nrow;
+    // With the learned map: a branch whose network proposals the yield gate
+    // passed less often than mapMinAccept over the probes is switched off
+    // (its projections take the exact map in the particle pass, with no
+    // queue and no map kernel)
+    if (b->rmap == kMapLearned) {
+        unsigned long long st[kMapStats];
+        BCHK(b, hipMemcpy(st, b->stats, sizeof(st), hipMemcpyDeviceToHost), "autotune");
+        int mask = 0;
+        const unsigned long long tA = st[3] - stats0[3], tF = st[4] - stats0[4];
+        const unsigned long long sA = st[5] - stats0[5], sF = st[6] - stats0[6];
+        if ((b->mapBranches & kMapTension) && tA + tF > 0 && (double)tA >= b->mapMinAccept * (double)(tA + tF))
+            mask |= kMapTension;
+        if ((b->mapBranches & kMapShear) && sA + sF > 0 && (double)sA >= b->mapMinAccept * (double)(sA + sF))
+            mask |= kMapShear;
+        b->tuneBranchStats[0] = tA; b->tuneBranchStats[1] = tF;
+        b->tuneBranchStats[2] = sA; b->tuneBranchStats[3] = sF;
+        if ((rc = icempm_batch_set_map_branches(b, mask))) return rc;
+    }
     // back to the checkpointed state, with the fastest setting
     if ((rc = icempm_batch_restore(b))) return rc;
     BCHK(b, hipMemcpy(b->stats, stats0, sizeof(stats0), hipMemcpyHostToDevice), "autotune");
@@ -2111,11 +2157,45 @@ int icempm_batch_map_stats(icempm_batch *b, unsigned long long *out, int reset) 
     if (!b || !out) return ICEMPM_E_ARG;
     if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "map_stats: batch not finalized");
     BCHK(b, hipStreamSynchronize(b->stream), "map_stats");
-    BCHK(b, hipMemcpy(out, b->stats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "map_stats");
+    unsigned long long all[kMapStats];
+    BCHK(b, hipMemcpy(all, b->stats, sizeof(all), hipMemcpyDeviceToHost), "map_stats");
+    std::memcpy(out, all, 3 * sizeof(unsigned long long));
     if (reset) {
-        const unsigned long long z3[3] = {0, 0, 0};
-        BCHK(b, hipMemcpy(b->stats, z3, sizeof(z3), hipMemcpyHostToDevice), "map_stats");
+        const unsigned long long z[kMapStats] = {};
+        BCHK(b, hipMemcpy(b->stats, z, sizeof(z), hipMemcpyHostToDevice), "map_stats");
     }
+    return ICEMPM_OK;
+}
+
+int icempm_batch_map_branch_stats(icempm_batch *b, unsigned long long *out, int reset) {
+    if (!b || !out) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "map_branch_stats: batch not finalized");
+    BCHK(b, hipStreamSynchronize(b->stream), "map_branch_stats");
+    unsigned long long all[kMapStats];
+    BCHK(b, hipMemcpy(all, b->stats, sizeof(all), hipMemcpyDeviceToHost), "map_branch_stats");
+    std::memcpy(out, all + 3, 4 * sizeof(unsigned long long));
+    if (reset) {
+        const unsigned long long z[kMapStats] = {};
+        BCHK(b, hipMemcpy(b->stats, z, sizeof(z), hipMemcpyHostToDevice), "map_branch_stats");
+    }
+    return ICEMPM_OK;
+}
+
+int icempm_batch_set_map_branches(icempm_batch *b, int mask) {
+    if (!b || mask < 0 || mask > (kMapTension | kMapShear)) return ICEMPM_E_ARG;
+    if (mask != b->mapBranches) {
+        BCHK(b, hipStreamSynchronize(b->stream), "set_map_branches");
+        drop_graph(b);   // the captured launches hold the mask (and the map kernel or not)
+        b->mapBranches = mask;
+    }
+    return ICEMPM_OK;
+}
+
+int icempm_batch_get_map_branches(const icempm_batch *b) { return b ? b->mapBranches : -1; }
+
+int icempm_batch_autotune_branch_stats(const icempm_batch *b, unsigned long long *out) {
+    if (!b || !out) return ICEMPM_E_ARG;
+    std::memcpy(out, b->tuneBranchStats, sizeof(b->tuneBranchStats));
     return ICEMPM_OK;
 }
 

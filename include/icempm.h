@@ -245,7 +245,8 @@ int icempm_batch_finalize(icempm_batch *b);
  * captures the loop once per n and replays it */
 int icempm_batch_run(icempm_batch *b, int n, int use_graph);
 /* device time of the last icempm_batch_run (HIP events on the batch stream;
- * a graph run is timed from its launch, after capture and instantiation) */
+ * a graph run is timed from its launch, after capture, instantiation and
+ * hipGraphUpload of the graph) */
 int icempm_batch_last_ms(const icempm_batch *b, float *ms);
 int icempm_batch_num_points(const icempm_batch *b);
 long icempm_batch_num_nodes(const icempm_batch *b);
@@ -269,9 +270,11 @@ int icempm_batch_error_flag(icempm_batch *b, int *flag);
 /* Tuning knobs: ICEMPM_BATCH_TUNE_CHUNK, points per chunk = threads per
  * workgroup of the particle pass (64, 128 or 256; default 256);
  * ICEMPM_BATCH_TUNE_RESORT, re-sort the points by cell every that many
- * substeps within a run (0, the default: never).  get returns -1 for an
- * unknown knob. */
-enum { ICEMPM_BATCH_TUNE_CHUNK = 0, ICEMPM_BATCH_TUNE_RESORT = 1 };
+ * substeps within a run (0, the default: never);
+ * ICEMPM_BATCH_TUNE_MAP_MIN_ACCEPT, the learned map's acceptance threshold
+ * in percent (default 50) below which icempm_batch_autotune switches the
+ * network off for a branch.  get returns -1 for an unknown knob. */
+enum { ICEMPM_BATCH_TUNE_CHUNK = 0, ICEMPM_BATCH_TUNE_RESORT = 1, ICEMPM_BATCH_TUNE_MAP_MIN_ACCEPT = 2 };
 int icempm_batch_set_tuning(icempm_batch *b, int knob, int value);
 int icempm_batch_get_tuning(const icempm_batch *b, int knob);
 /* stable re-sort of every case's points by current cell (reads keep returning
@@ -315,8 +318,24 @@ int icempm_batch_read_harvest(icempm_batch *b, float *X, float *T, int cap,
                               unsigned long long *total);
 /* out[0] points accepted from the MLP, out[1] points that fell back to the
  * exact map, out[2] crushed points inside the surface (no projection), since
- * the last reset; reset != 0 zeroes the counters after reading */
+ * the last reset; reset != 0 zeroes the counters after reading (these and the
+ * per-branch ones) */
 int icempm_batch_map_stats(icempm_batch *b, unsigned long long *out, int reset);
+/* The network's proposals per branch of the map since the last reset:
+ * out[0] tension accepted, out[1] tension fell back, out[2] shear accepted,
+ * out[3] shear fell back (points the network was asked about) */
+int icempm_batch_map_branch_stats(icempm_batch *b, unsigned long long *out, int reset);
+/* Branches the network proposes for (ICEMPM_MAP_TENSION | ICEMPM_MAP_SHEAR,
+ * both by default); a switched-off branch takes the exact map in the particle
+ * pass, with no queue, and with both off no map kernel runs.  In learned mode
+ * icempm_batch_autotune switches off each branch whose proposals passed the
+ * yield gate less often than ICEMPM_BATCH_TUNE_MAP_MIN_ACCEPT percent over
+ * its probes; autotune_branch_stats returns those probes' counts (as
+ * map_branch_stats). */
+enum { ICEMPM_MAP_TENSION = 1, ICEMPM_MAP_SHEAR = 2 };
+int icempm_batch_set_map_branches(icempm_batch *b, int mask);
+int icempm_batch_get_map_branches(const icempm_batch *b);
+int icempm_batch_autotune_branch_stats(const icempm_batch *b, unsigned long long *out);
 
 /* -- the learned return map (no counterpart) -----------------------------
  * A 5 -> 64 -> 64 -> 3 MLP (tanh hidden layers) with BF16 weights and

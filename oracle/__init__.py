@@ -19,8 +19,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+FAST_LIB_PATH = os.path.join(HERE, "liboracle_fast.so")   # the CPU-baseline timing build
 
 _lib = None
+_lib_fast = None
 
 
 def build() -> str:
@@ -29,12 +31,25 @@ def build() -> str:
 
 
 def lib() -> ctypes.CDLL:
+    """The parity build (-O2, no contraction)."""
     global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise ImportError("oracle/liboracle.so is not built (make -C oracle)")
-    L = ctypes.CDLL(LIB_PATH)
+    if _lib is None:
+        _lib = _load(LIB_PATH)
+    return _lib
+
+
+def lib_fast() -> ctypes.CDLL:
+    """The timing build of the same source (-O3, AVX2), for the CPU baseline."""
+    global _lib_fast
+    if _lib_fast is None:
+        _lib_fast = _load(FAST_LIB_PATH)
+    return _lib_fast
+
+
+def _load(path: str) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"oracle/{os.path.basename(path)} is not built (make -C oracle)")
+    L = ctypes.CDLL(path)
     dp = POINTER(c_double)
     ip = POINTER(c_int)
     L.oracle_simparams_size.restype = c_int
@@ -60,7 +75,6 @@ def lib() -> ctypes.CDLL:
     L.oracle_stable_order.argtypes = [ip, c_int, ip]
     L.oracle_mlp_loss_grad.argtypes = [dp, dp, dp, c_int, dp, dp]
     L.oracle_mlp_loss_grad.restype = c_double
-    _lib = L
     return L
 
 
@@ -149,12 +163,13 @@ def g2p(prms, pts: np.ndarray, grid: np.ndarray, record: bool) -> int:
 
 
 def run(prms, pts: np.ndarray, grid: np.ndarray, bins: np.ndarray, nsub: int,
-        nthreads: int = 1) -> tuple[np.ndarray, int]:
-    """nsub substeps with Model::Step semantics (prms is advanced)."""
+        nthreads: int = 1, fast: bool = False) -> tuple[np.ndarray, int]:
+    """nsub substeps with Model::Step semantics (prms is advanced); fast
+    selects the timing build."""
     _check_buffers(prms, pts, grid, bins)
     hist = np.zeros((max(nsub, 1), 2), dtype=np.float64)
-    err = lib().oracle_run(ctypes.byref(prms), _dp(pts), _dp(grid), _dp(bins), nsub, _dp(hist),
-                           nthreads)
+    L = lib_fast() if fast else lib()
+    err = L.oracle_run(ctypes.byref(prms), _dp(pts), _dp(grid), _dp(bins), nsub, _dp(hist), nthreads)
     return hist[:nsub], err
 
 

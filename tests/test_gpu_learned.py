@@ -161,6 +161,34 @@ def test_trained_map_force_error_and_gain(gpu_available):
     assert rate[("trained", 1e-2)] > 0.5
     assert rate[("trained", 1e-2)] > 10 * rate[("untrained", 1e-2)]
 
+    # The gate decides where the network runs at all: the autotuner counts,
+    # per branch of the map, how often the network's proposals passed the
+    # yield gate over its probes, and switches a branch below 50% off (its
+    # projections then take the exact map in the particle pass: no queue, no
+    # map kernel).  At 1e-4 both branches go off and the run is the exact
+    # map's; at 1e-2 the network keeps the branches it serves.
+    for tol in (1e-4, 1e-2):
+        b = build_batch(cs, BATCH_F64)
+        b.set_learned(params, norm, tol=tol)
+        b.autotune(probe_steps=100)
+        ta, tf, sa, sf = b.autotune_branch_stats()
+        mask = b.map_branches()
+        b.map_stats(reset=True)
+        b.run(NSUB)
+        assert b.error_flag() == 0
+        acc, fb, ins = b.map_stats()
+        errs = [rel_l2(h, r) for h, r in zip(histories(b, NSUB), ref_hist)]
+        print(f"gated tol={tol:g}: probe acceptance tension {ta}/{ta + tf}, shear {sa}/{sa + sf} -> branches "
+              f"{mask}; run: accepted {acc} fallback {fb}, max force rel L2 {max(errs):.3e}", flush=True)
+        assert ta + tf + sa + sf > 0
+        if tol == 1e-4:
+            assert mask == 0 and acc == 0
+            assert max(errs) <= 1e-9          # the exact map's run, to run-to-run rounding
+        else:
+            assert mask != 0 and acc > 0
+            assert max(errs) <= 1e-1
+        b.close()
+
 
 def test_train_on_50m_harvested_pairs(gpu_available):
     """The harvest and training at full scale: 50M (trial, projected) pairs

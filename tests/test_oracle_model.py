@@ -122,3 +122,24 @@ def test_ingest_places_block_and_indenter():
     assert p.ParticleMass == pytest.approx(case.volume / n * p.Density, rel=1e-7)
     assert (pts[8, :n] == 1).all() and (pts[11, :n] == 1).all() and (pts[3, :n] == 1).all()
     assert keys.shape == (n,)
+
+
+def test_timing_build_equals_parity_build():
+    """The CPU baseline's -O3/AVX2 build of the oracle gives the parity
+    build's results bit for bit (both keep a*b+c uncontracted, and neither
+    reassociates)."""
+    import ctypes
+    from icempm_amd import INDENT_TINY, SimParams, configure_params, lattice_points
+    p = SimParams()
+    configure_params(p, INDENT_TINY)
+    xy, g = lattice_points(INDENT_TINY)
+    pts = oracle.ingest(p, xy, g, INDENT_TINY.volume, pitch=2432)
+    p.nGridPitch = (p.GridTotal + 63) // 64 * 64
+    q = SimParams()
+    ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(SimParams))
+    pts2 = pts.copy()
+    h1, e1 = oracle.run(p, pts, oracle.new_grid(p), np.zeros(200), 300)
+    h2, e2 = oracle.run(q, pts2, oracle.new_grid(q), np.zeros(200), 300, nthreads=1, fast=True)
+    assert e1 == 0 and e2 == 0
+    assert np.array_equal(h1, h2)
+    assert np.array_equal(pts.view(np.uint64), pts2.view(np.uint64))
