@@ -158,7 +158,7 @@ HOST_EXPORTS = [
 H5_EXPORTS = [
     "icempm_model_load_raw_points_file", "icempm_model_save_snapshot", "icempm_model_save_pq",
     "icempm_model_read_snapshot",
-    "icempm_h5_write_raw_points",
+    "icempm_h5_write_raw_points", "icempm_h5_write_snapshot",
 ]
 
 _hip = None
@@ -313,6 +313,8 @@ def h5():
     _sig(L, "icempm_model_read_snapshot", c_int, P, c_char_p)
     _sig(L, "icempm_h5_write_raw_points", c_int, c_char_p, POINTER(c_float), POINTER(c_int16), c_int,
          c_float, c_int)
+    _sig(L, "icempm_h5_write_snapshot", c_int, c_char_p, c_void_p, c_size_t, POINTER(c_double),
+         POINTER(c_double))
     _h5 = L
     return L
 

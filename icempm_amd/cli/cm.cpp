@@ -15,12 +15,15 @@
 // "cases" array lists configuration files (paths relative to the manifest),
 // with optional "precision" ("f32" or "f64") and "output" (directory).
 // Each configuration's raw point file is ingested as the single-case path
-// does (LoadRawPoints); every case must have a uniform material
-// (GrainVariability 0) and all share one time step.  Output cycles are
-// UpdateEveryNthStep substeps of the first case; the run ends after
-// --cycles cycles or when the first case reaches its SimulationEndTime.
-// Each cycle appends every case's per-substep total indenter force to
-// DIR/case_K.csv (substep, time, fx, fy) and one line to DIR/batch.csv.
+// does (LoadRawPoints), grains included; all cases share one time step.
+// Output cycles are UpdateEveryNthStep substeps of the first case; the run
+// ends after --cycles cycles or when the first case reaches its
+// SimulationEndTime.  Each cycle writes every case's snapshot in the
+// single-case format, DIR/case_K/snapshots/dNNNNN.h5 (Indenter_2D: the
+// cycle's force bins; Params with SimulationStep, SimulationTime and the
+// indenter position; Points in the ingest order), appends the case's
+// per-substep total indenter force to DIR/case_K.csv (substep, time, fx, fy)
+// and one line to DIR/batch.csv.
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -34,6 +37,7 @@
 #include <vector>
 
 #include "icempm.h"
+#include "icempm_h5.h"
 #include "model.h"
 #include "snapshotmanager.h"
 
@@ -178,7 +182,7 @@ int run_batch(const std::string &manifestPath, std::string out, bool f32, long m
     summary << "cycle,substeps,ms,particle_updates_per_s\n";
     std::cerr << "cm: batch of " << prms.size() << " cases, " << total << " points, " << (useF32 ? "f32" : "f64")
               << " storage, " << cycles << " cycles of " << N << " substeps\n";
-    std::vector<double> hist(2 * (size_t)N);
+    std::vector<double> hist(2 * (size_t)N), bins, soa;
     double t = 0;   // by repeated addition, as Model::Step does
     long step = 0;
     for (long cy = 1; cy <= cycles; ++cy) {
@@ -196,6 +200,17 @@ int run_batch(const std::string &manifestPath, std::string out, bool f32, long m
             check(b, icempm_batch_force_history(b, (int)c, hist.data(), N), "force_history");
             for (int i = 0; i < N; ++i)
                 csv[c] << step + i + 1 << "," << times[i] << "," << hist[2 * i] << "," << hist[2 * i + 1] << "\n";
+            icy::SimParams &P = prms[c];
+            P.SimulationStep = (int)(step + N);
+            P.SimulationTime = t;
+            P.indenter_x = P.indenter_x_initial + t * P.IndVelocity;
+            bins.assign(2 * (size_t)P.n_indenter_subdivisions, 0.0);
+            soa.assign(icy::SimParams::nPtsArrays * (size_t)P.nPtsPitch, 0.0);
+            check(b, icempm_batch_force_bins(b, (int)c, bins.data(), P.n_indenter_subdivisions), "force_bins");
+            check(b, icempm_batch_read_points(b, (int)c, soa.data(), P.nPtsPitch), "read_points");
+            const std::string dir = (std::filesystem::path(out) / ("case_" + std::to_string(c))).string();
+            if (icempm_h5_write_snapshot(dir.c_str(), &P, sizeof(P), bins.data(), soa.data()) != 0)
+                throw std::runtime_error("writing the snapshot of case " + std::to_string(c) + " failed");
         }
         step += N;
         summary << cy << "," << N << "," << ms << "," << (ms > 0 ? total * (double)N / (ms * 1e-3) : 0.0) << "\n";

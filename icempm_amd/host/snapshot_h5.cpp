@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <filesystem>
 #include <stdexcept>
 #include <string>
@@ -65,12 +66,8 @@ hid_t write_dataset(hid_t file, const char *name, hid_t type, int rank, const hs
     return ds;
 }
 
-}  // namespace
-
-void icy::SnapshotManager::SaveParametersAsAttributes(long long dataset)
-{
-    const hid_t ds = (hid_t)dataset;
-    const icy::SimParams &P = model->prms;
+// the 14 attributes of SaveParametersAsAttributes (snapshotmanager.cpp:107-145)
+void parameter_attributes(hid_t ds, const icy::SimParams &P) {
     const int64_t pitch = (int64_t)P.nPtsPitch;
     scalar_attr(ds, "indenter_x", H5T_NATIVE_DOUBLE, P.indenter_x);
     scalar_attr(ds, "indenter_y", H5T_NATIVE_DOUBLE, P.indenter_y);
@@ -88,27 +85,40 @@ void icy::SnapshotManager::SaveParametersAsAttributes(long long dataset)
     scalar_attr(ds, "Volume", H5T_NATIVE_DOUBLE, P.Volume);
 }
 
-// snapshotmanager.cpp:60-105: <dir>/snapshots/dNNNNN.h5 with the force bins
-// (Indenter_2D), the raw parameter struct (Params, with attributes) and the
-// pitched point SoA (Points), from the pinned readback buffers.
+// snapshotmanager.cpp:60-105: <dir>/snapshots/dNNNNN.h5 (NNNNN the
+// animation frame of P) with the force bins (Indenter_2D), the raw parameter
+// struct (Params, with attributes) and the pitched point SoA (Points)
+void write_snapshot_file(const std::string &outputDirectory, const std::string &snapshotsDir,
+                         const icy::SimParams &P, const double *indenter, const double *points) {
+    ensure_dir(outputDirectory);
+    ensure_dir(outputDirectory + "/" + snapshotsDir);
+    char fileName[20];
+    std::snprintf(fileName, sizeof(fileName), "d%05d.h5", P.SimulationStep / P.UpdateEveryNthStep);   // AnimationFrameNumber
+    const std::string path = outputDirectory + "/" + snapshotsDir + "/" + fileName;
+
+    H file(check_id(H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), path.c_str()), H5Fclose);
+    const hsize_t nInd = (hsize_t)P.n_indenter_subdivisions * 2;
+    H ind(write_dataset(file, "Indenter_2D", H5T_NATIVE_DOUBLE, 1, &nInd, indenter), H5Dclose);
+    const hsize_t nPrm = sizeof(icy::SimParams);
+    H prm(write_dataset(file, "Params", H5T_NATIVE_B8, 1, &nPrm, &P), H5Dclose);
+    parameter_attributes(prm.id, P);
+    const hsize_t nPts = (hsize_t)P.nPtsPitch * icy::SimParams::nPtsArrays;
+    H pts(write_dataset(file, "Points", H5T_NATIVE_DOUBLE, 1, &nPts, points), H5Dclose);
+}
+
+}  // namespace
+
+void icy::SnapshotManager::SaveParametersAsAttributes(long long dataset)
+{
+    parameter_attributes((hid_t)dataset, model->prms);
+}
+
+// snapshotmanager.cpp:60-105, from the pinned readback buffers
 void icy::SnapshotManager::SaveSnapshot(std::string outputDirectory)
 {
     if (!model) throw std::runtime_error("SnapshotManager: no model");
-    ensure_dir(outputDirectory);
-    ensure_dir(outputDirectory + "/" + directory_snapshots);
-    char fileName[20];
-    std::snprintf(fileName, sizeof(fileName), "d%05d.h5", model->prms.AnimationFrameNumber());
-    const std::string path = outputDirectory + "/" + directory_snapshots + "/" + fileName;
-
-    H file(check_id(H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), path.c_str()), H5Fclose);
-    const hsize_t nInd = (hsize_t)model->prms.n_indenter_subdivisions * 2;
-    H ind(write_dataset(file, "Indenter_2D", H5T_NATIVE_DOUBLE, 1, &nInd,
-                        model->gpu.host_side_indenter_force_accumulator), H5Dclose);
-    const hsize_t nPrm = sizeof(icy::SimParams);
-    H prm(write_dataset(file, "Params", H5T_NATIVE_B8, 1, &nPrm, &model->prms), H5Dclose);
-    SaveParametersAsAttributes(prm.id);
-    const hsize_t nPts = (hsize_t)model->prms.nPtsPitch * icy::SimParams::nPtsArrays;
-    H pts(write_dataset(file, "Points", H5T_NATIVE_DOUBLE, 1, &nPts, model->gpu.tmp_transfer_buffer), H5Dclose);
+    write_snapshot_file(outputDirectory, directory_snapshots, model->prms,
+                        model->gpu.host_side_indenter_force_accumulator, model->gpu.tmp_transfer_buffer);
 }
 
 // snapshotmanager.cpp:14-56: <dir>/pq/pqNNNNN.h5 with (p, q) of the intact
@@ -246,6 +256,21 @@ int icempm_model_save_snapshot(icempm_model *m, const char *dir) {
 int icempm_model_save_pq(icempm_model *m, const char *dir) {
     if (!m || !dir) return ICEMPM_E_ARG;
     return icempm_guarded(m, [&] { m->snapshot.SavePQ(std::string(dir)); });
+}
+
+int icempm_h5_write_snapshot(const char *dir, const void *simparams, size_t nbytes, const double *indenter_force,
+                             const double *points) {
+    if (!dir || !simparams || nbytes != sizeof(icy::SimParams) || !indenter_force || !points)
+        return ICEMPM_E_ARG;
+    icy::SimParams P;
+    std::memcpy(&P, simparams, sizeof(P));
+    if (P.n_indenter_subdivisions <= 0 || P.nPtsPitch <= 0 || P.UpdateEveryNthStep <= 0) return ICEMPM_E_ARG;
+    try {
+        write_snapshot_file(dir, "snapshots", P, indenter_force, points);
+        return ICEMPM_OK;
+    } catch (const std::exception &) {
+        return ICEMPM_E_IO;
+    }
 }
 
 int icempm_h5_write_raw_points(const char *path, const float *xy, const int16_t *grains, int n, float volume,

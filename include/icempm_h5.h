@@ -7,6 +7,7 @@
 #ifndef ICEMPM_H5_H
 #define ICEMPM_H5_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "icempm_host.h"
@@ -26,6 +27,14 @@ int icempm_model_load_raw_points_file(icempm_model *m, const char *path);
  * SaveParametersAsAttributes, snapshotmanager.cpp:107-145), Points
  * f64[16 nPtsPitch] */
 int icempm_model_save_snapshot(icempm_model *m, const char *dir);
+/* The same snapshot file from caller-held data (no counterpart in the
+ * reference; for cm's batch mode, which has no icy::Model per case):
+ * simparams is a full icy::SimParams (nbytes == its size) whose
+ * SimulationStep, SimulationTime and indenter_x/y describe the state;
+ * indenter_force holds 2 n_indenter_subdivisions doubles and points the
+ * pitched SoA, 16 nPtsPitch doubles.  Writes <dir>/snapshots/dNNNNN.h5. */
+int icempm_h5_write_snapshot(const char *dir, const void *simparams, size_t nbytes,
+                             const double *indenter_force, const double *points);
 /* Resume from a snapshot written by icempm_model_save_snapshot (the
  * reference's ReadSnapshot, snapshotmanager.cpp:147-184, is commented out):
  * parameters, arrays and points restored and uploaded; stepping continues

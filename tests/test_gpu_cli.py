@@ -105,8 +105,9 @@ def run_cm_batch(tmp_path, cases, f32):
     return refs, N, hists
 
 
-def oracle_histories(refs, N):
-    """The oracle's ingest and N substeps of each case alone."""
+def oracle_runs(refs, N):
+    """The oracle's ingest and N substeps of each case alone: (force history,
+    force bins, points, parameters after the run) per case."""
     out = []
     for p, xy, g, vol in refs:
         pts = oracle.ingest(p, xy, g, vol, (xy.shape[0] + 63) // 64 * 64)
@@ -115,21 +116,46 @@ def oracle_histories(refs, N):
         bins = np.zeros(2 * p.n_indenter_subdivisions)
         href, err = oracle.run(p, pts, grid, bins, N)
         assert err == 0 and np.abs(href).sum() > 0
-        out.append(href)
+        out.append((href, bins, pts, p))
     return out
+
+
+def check_case_snapshots(out, runs, N, tol_bins, tol_x, tmp_path):
+    """cm --batch's per-case snapshot after the first cycle against the oracle:
+    frame d00001.h5, SimulationStep N and SimulationTime as Model::Step sums
+    them, the cycle's force bins (Indenter_2D) and the points' positions."""
+    for k, (href, bins, pts, p) in enumerate(runs):
+        snap = out / f"case_{k}" / "snapshots" / "d00001.h5"
+        assert snap.exists(), snap
+        prm = SimParams.from_buffer_copy(binary(snap, "/Params", "u1", tmp_path).tobytes())
+        assert prm.SimulationStep == N and prm.SimulationTime == p.SimulationTime
+        assert prm.indenter_x == prm.indenter_x_initial + prm.SimulationTime * prm.IndVelocity
+        ind = binary(snap, "/Indenter_2D", "<f8", tmp_path)
+        assert ind.shape == bins.shape and np.abs(bins).sum() > 0
+        e = np.linalg.norm(ind - bins) / np.linalg.norm(bins)
+        dev = binary(snap, "/Points", "<f8", tmp_path).reshape(16, -1)
+        n = p.nPts
+        dx = np.abs(dev[4:6, :n] - pts[4:6, :n]).max()
+        print(f"case {k}: snapshot force bins rel L2 {e:.2e}, positions {dx:.2e} m", flush=True)
+        assert e <= tol_bins and dx <= tol_x, (k, e, dx)
 
 
 def test_cm_batch_manifest_matches_oracle(gpu_available, tmp_path):
     """cm --batch: configurations listed in a manifest run as one batch (FP64
     storage); each case's per-substep indenter force over one output cycle
-    (case_K.csv) equals the oracle run of that case alone within 1e-6."""
+    (case_K.csv) and its snapshot's force bins equal the oracle run of that
+    case alone within 1e-6 relative L2 (atomic summation order; measured
+    ~2e-12), its positions within 1e-12 m (measured ~2e-15)."""
     from icempm_amd.batch import batch_cases
     print("cli: cm --batch, 3 cases, 1 cycle f64", flush=True)
     refs, N, hists = run_cm_batch(tmp_path, batch_cases(3, 1500, 4000, seed=21), False)
-    for k, (href, h) in enumerate(zip(oracle_histories(refs, N), hists)):
+    runs = oracle_runs(refs, N)
+    for k, (r, h) in enumerate(zip(runs, hists)):
+        href = r[0]
         e = np.linalg.norm(h - href) / np.linalg.norm(href)
         print(f"case {k}: force rel L2 {e:.2e}", flush=True)
         assert e <= 1e-6, (k, e)
+    check_case_snapshots(tmp_path / "out", runs, N, 1e-6, 1e-12, tmp_path)
 
 
 def test_cm_batch_manifest_f32(gpu_available, tmp_path):
@@ -137,16 +163,21 @@ def test_cm_batch_manifest_f32(gpu_available, tmp_path):
     cycle stays within the FP32 bar of the oracle (1e-3 relative L2; measured
     ~4e-6), and equals the library's FP32 batch run of the same cases
     (icempm_amd.batch) to the run-to-run reproducibility of the batched pass
-    (hardware atomics; 1e-4)."""
+    (hardware atomics; 1e-4); its snapshots hold the cycle's force bins within
+    1e-5 of the oracle's (measured ~2e-7) and positions within 1e-9 m
+    (measured ~1e-10)."""
     from icempm_amd._lib import BATCH_F32
     from icempm_amd.batch import batch_cases, build_batch
     print("cli: cm --batch, 3 cases, 1 cycle f32", flush=True)
     cases = batch_cases(3, 1500, 4000, seed=21)
     refs, N, hists = run_cm_batch(tmp_path, cases, True)
-    for k, (href, h) in enumerate(zip(oracle_histories(refs, N), hists)):
+    runs = oracle_runs(refs, N)
+    for k, (r, h) in enumerate(zip(runs, hists)):
+        href = r[0]
         e = np.linalg.norm(h - href) / np.linalg.norm(href)
         print(f"case {k}: force rel L2 vs oracle {e:.2e}", flush=True)
         assert e <= 1e-3, (k, e)
+    check_case_snapshots(tmp_path / "out", runs, N, 1e-5, 1e-9, tmp_path)
     b = build_batch(cases, BATCH_F32)
     b.run(N)
     assert b.error_flag() == 0
