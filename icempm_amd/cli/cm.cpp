@@ -8,12 +8,13 @@
 // end time or an out-of-grid point.
 //
 // usage: cm <config.json> [--snapshots DIR] [--ordered] [--pq] [--cycles N]
-//        cm --batch <manifest.json> [--out DIR] [--f32] [--cycles N]
+//        cm --batch <manifest.json> [--out DIR] [--f32 | --ordered] [--cycles N]
 //
 // The batch mode (no counterpart in the reference) runs many configurations
 // as one batch on the batched engine: the manifest is a JSON object whose
 // "cases" array lists configuration files (paths relative to the manifest),
-// with optional "precision" ("f32" or "f64") and "output" (directory).
+// with optional "precision" ("f64", "f32" or "ordered") and "output"
+// (directory).
 // Each configuration's raw point file is ingested as the single-case path
 // does (LoadRawPoints), grains included; all cases share one time step.
 // Output cycles are UpdateEveryNthStep substeps of the first case; the run
@@ -23,13 +24,20 @@
 // cycle's force bins; Params with SimulationStep, SimulationTime and the
 // indenter position; Points in the ingest order), appends the case's
 // per-substep total indenter force to DIR/case_K.csv (substep, time, fx, fy)
-// and one line to DIR/batch.csv.
+// and one line to DIR/batch.csv.  With --ordered (or "precision": "ordered")
+// the cases run one after another on the single-case engine in ORDERED mode
+// instead (the batched engine sums with hardware atomics and has no such
+// mode): the same files, each snapshot bit for bit the one `cm --ordered`
+// writes for that configuration; every case then needs the first case's
+// time step and UpdateEveryNthStep.
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -45,7 +53,7 @@ namespace {
 
 const char *kUsage =
     "usage: cm <config.json> [--snapshots DIR] [--ordered] [--pq] [--cycles N]\n"
-    "       cm --batch <manifest.json> [--out DIR] [--f32] [--cycles N]\n";
+    "       cm --batch <manifest.json> [--out DIR] [--f32 | --ordered] [--cycles N]\n";
 
 // The "cases" string array and the "precision" / "output" strings of a
 // manifest (a JSON object; other keys are ignored).
@@ -122,6 +130,8 @@ Manifest parse_manifest(const std::string &path) {
         throw std::runtime_error("manifest: expected ',' or '}'");
     }
     if (m.cases.empty()) throw std::runtime_error("manifest: no cases");
+    if (m.precision != "f64" && m.precision != "f32" && m.precision != "ordered")
+        throw std::runtime_error("manifest: precision must be \"f64\", \"f32\" or \"ordered\"");
     return m;
 }
 
@@ -131,9 +141,90 @@ void check(icempm_batch *b, int rc, const char *what) {
                                  " (code " + std::to_string(rc) + ")");
 }
 
-int run_batch(const std::string &manifestPath, std::string out, bool f32, long maxCycles) {
+std::filesystem::path case_config(const std::filesystem::path &base, const std::string &c) {
+    return std::filesystem::path(c).is_absolute() ? std::filesystem::path(c) : base / c;
+}
+
+// The manifest's cases on the single-case engine in ORDERED mode, one model
+// per case, stepped case after case each cycle by Model::Step.
+int run_batch_ordered(const Manifest &man, const std::filesystem::path &base, const std::string &out,
+                      long maxCycles) {
+    std::vector<std::unique_ptr<icy::Model>> models;
+    std::vector<std::unique_ptr<icy::SnapshotManager>> snaps;
+    long total = 0;
+    for (const std::string &c : man.cases) {
+        const std::filesystem::path cfg = case_config(base, c);
+        models.push_back(std::make_unique<icy::Model>());
+        snaps.push_back(std::make_unique<icy::SnapshotManager>());
+        icy::Model &model = *models.back();
+        model.gpu.initialize();
+        model.Reset();
+        model.gpu.set_mode(ICEMPM_MODE_ORDERED);
+        snaps.back()->model = &model;
+        const std::string raw = model.prms.ParseFile(cfg.string());
+        snaps.back()->LoadRawPoints(raw);
+        const icy::SimParams &P0 = models[0]->prms;
+        if (model.prms.InitialTimeStep != P0.InitialTimeStep || model.prms.UpdateEveryNthStep != P0.UpdateEveryNthStep)
+            throw std::runtime_error(cfg.string() + ": time step or UpdateEveryNthStep differs from the first case's");
+        total += model.prms.nPts;
+        std::cerr << "cm: case " << models.size() - 1 << " " << cfg.string() << ": " << model.prms.nPts
+                  << " points, grid " << model.prms.GridX << " x " << model.prms.GridY << "\n";
+    }
+    const int N = models[0]->prms.UpdateEveryNthStep;
+    if (N <= 0) throw std::runtime_error("UpdateEveryNthStep must be positive");
+    long cycles = maxCycles;
+    if (cycles < 0) cycles = (long)std::ceil(models[0]->prms.SimulationEndTime / (models[0]->prms.InitialTimeStep * N));
+    std::filesystem::create_directories(out);
+    std::vector<std::ofstream> csv;
+    for (size_t c = 0; c < models.size(); ++c) {
+        csv.emplace_back(std::filesystem::path(out) / ("case_" + std::to_string(c) + ".csv"));
+        csv.back() << "substep,time,fx,fy\n";
+        csv.back().precision(17);
+    }
+    std::ofstream summary(std::filesystem::path(out) / "batch.csv");
+    summary << "cycle,substeps,ms,particle_updates_per_s\n";
+    std::cerr << "cm: " << models.size() << " cases, " << total << " points, ORDERED single-case engine, " << cycles
+              << " cycles of " << N << " substeps\n";
+    std::vector<double> hist(2 * (size_t)N);
+    for (long cy = 1; cy <= cycles; ++cy) {
+        const auto t0 = std::chrono::steady_clock::now();
+        bool bad = false;
+        for (size_t c = 0; c < models.size(); ++c) {
+            icy::Model &model = *models[c];
+            const double tStart = model.prms.SimulationTime;
+            const int step0 = model.prms.SimulationStep;
+            model.Step();
+            // no transfer callback: the readback's completion releases the cycle
+            model.processing_current_cycle_data.lock();
+            model.processing_current_cycle_data.unlock();
+            if (icempm_force_history(model.gpu.context(), hist.data(), N) != 0)
+                throw std::runtime_error("force_history of case " + std::to_string(c) + " failed");
+            double t = tStart;
+            for (int i = 0; i < N; ++i) {
+                t += model.prms.InitialTimeStep;
+                csv[c] << step0 + i + 1 << "," << t << "," << hist[2 * i] << "," << hist[2 * i + 1] << "\n";
+            }
+            snaps[c]->SaveSnapshot((std::filesystem::path(out) / ("case_" + std::to_string(c))).string());
+            bad = bad || model.gpu.error_code != 0;
+        }
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        summary << cy << "," << N << "," << ms << "," << (ms > 0 ? total * (double)N / (ms * 1e-3) : 0.0) << "\n";
+        std::cerr << "cm: cycle " << cy << " of " << cycles << ", " << ms / N << " ms per substep (host clock)\n";
+        if (bad) {
+            std::cerr << "cm: a point left the grid\n";
+            return 3;
+        }
+    }
+    return 0;
+}
+
+int run_batch(const std::string &manifestPath, std::string out, bool f32, bool ordered, long maxCycles) {
     const Manifest man = parse_manifest(manifestPath);
     if (out.empty()) out = man.output.empty() ? "cm_batch" : man.output;
+    if (ordered || man.precision == "ordered") {
+        if (f32) throw std::runtime_error("--ordered runs FP64 storage; --f32 does not apply");
+        return run_batch_ordered(man, std::filesystem::path(manifestPath).parent_path(), out, maxCycles);
+    }
     const bool useF32 = f32 || man.precision == "f32";
     const std::filesystem::path base = std::filesystem::path(manifestPath).parent_path();
     icempm_batch *b = nullptr;
@@ -146,7 +237,7 @@ int run_batch(const std::string &manifestPath, std::string out, bool f32, long m
     std::vector<icy::SimParams> prms;
     long total = 0;
     for (const std::string &c : man.cases) {
-        const std::filesystem::path cfg = std::filesystem::path(c).is_absolute() ? std::filesystem::path(c) : base / c;
+        const std::filesystem::path cfg = case_config(base, c);
         // the single-case ingest: parse, load and sort the raw points on the device, read them back
         icy::Model model;
         icy::SnapshotManager snapshot;
@@ -246,12 +337,12 @@ int main(int argc, char **argv)
         }
     }
     if (!manifest.empty()) {
-        if (!config.empty() || ordered || pq) {
+        if (!config.empty() || pq) {
             std::cerr << kUsage;
             return 2;
         }
         try {
-            const int rc = run_batch(manifest, out, f32, maxCycles);
+            const int rc = run_batch(manifest, out, f32, ordered, maxCycles);
             if (rc) return rc;
         } catch (const std::exception &e) {
             std::cerr << "cm: " << e.what() << "\n";

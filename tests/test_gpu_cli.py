@@ -77,17 +77,23 @@ def case_config(case, raw):
     }
 
 
-def run_cm_batch(tmp_path, cases, f32):
+def run_cm_batch(tmp_path, cases, f32, precision=None, grain_seed=None):
+    """cm --batch over the cases, one cycle; with grain_seed, every case but
+    the last gets seeded grain ids in [0, 1000)."""
     names, refs = [], []
+    rng = np.random.default_rng(grain_seed) if grain_seed is not None else None
     for k, case in enumerate(cases):
         xy, g = lattice_points(case)
+        if rng is not None and k < len(cases) - 1:
+            g = rng.integers(0, 1000, size=g.shape[0]).astype(np.int16)
         raw = tmp_path / f"raw{k}.h5"
         write_raw_points(str(raw), xy, g, case.volume, 7)
         p, cfg = case_config(case, raw)
         (tmp_path / f"c{k}.json").write_text(json.dumps(cfg))
         names.append(f"c{k}.json")
         refs.append((p, xy, g, case.volume))
-    (tmp_path / "m.json").write_text(json.dumps({"cases": names, "precision": "f32" if f32 else "f64"}))
+    (tmp_path / "m.json").write_text(json.dumps({"cases": names,
+                                                 "precision": precision or ("f32" if f32 else "f64")}))
     out = tmp_path / "out"
     r = subprocess.run([CLI_BIN, "--batch", str(tmp_path / "m.json"), "--out", str(out), "--cycles", "1"],
                        stdin=subprocess.DEVNULL, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
@@ -187,3 +193,30 @@ def test_cm_batch_manifest_f32(gpu_available, tmp_path):
         print(f"case {k}: force rel L2 vs library {e:.2e}", flush=True)
         assert e <= 1e-4, (k, e)
     b.close()
+
+
+def test_cm_batch_ordered_is_bit_exact(gpu_available, tmp_path):
+    """cm --batch with "precision": "ordered": the manifest's cases on the
+    single-case ORDERED engine, per-grain strengths (GrainVariability 0.5,
+    grain ids 0..999) on two of them; each case's snapshot (Points,
+    Indenter_2D, step and time) and per-substep force equal the oracle run of
+    that case alone bit for bit, as `cm --ordered` does."""
+    import dataclasses
+
+    from icempm_amd.batch import batch_cases
+    print("cli: cm --batch ordered, 3 cases, 1 cycle", flush=True)
+    cases = batch_cases(3, 1500, 4000, seed=23)
+    cases = [dataclasses.replace(c, grain_variability=0.5) if k < 2 else c for k, c in enumerate(cases)]
+    refs, N, hists = run_cm_batch(tmp_path, cases, False, precision="ordered", grain_seed=5)
+    runs = oracle_runs(refs, N)
+    crushed = []
+    for k, ((href, bins, pts, p), h) in enumerate(zip(runs, hists)):
+        assert np.array_equal(h, href), k
+        snap = tmp_path / "out" / f"case_{k}" / "snapshots" / "d00001.h5"
+        prm = SimParams.from_buffer_copy(binary(snap, "/Params", "u1", tmp_path).tobytes())
+        assert prm.SimulationStep == N and prm.SimulationTime == p.SimulationTime
+        assert np.array_equal(binary(snap, "/Indenter_2D", "<f8", tmp_path), bins), k
+        dev = binary(snap, "/Points", "<f8", tmp_path).reshape(16, -1)
+        assert np.array_equal(dev.view(np.uint64), pts.view(np.uint64)), k
+        crushed.append(int(pts[0].view(np.uint8)[:p.nPts].sum()))
+    print("crushed points per case", crushed, flush=True)
