@@ -441,7 +441,8 @@ def run_batch_learned(args) -> dict:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_batch_particles<F32,G2P,P2G,learned> (fused G2P + P2G, all cases)",
+            "kernel": (f"k_batch_particles<F32,G2P,P2G,{'learned' if branches else 'exact'}> "
+                       "(fused G2P + P2G, all cases)"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -469,10 +470,12 @@ def run_batch_learned(args) -> dict:
             "speedup_vs_batched_f64": value / f64_pups,
             "force_rel_l2_max_learned_vs_f64_exact": max(errs),
             "force_rel_l2_max_f32_exact_vs_f64_exact": max(errs32),
+            # with both branches off the pass is the exact map's, which keeps
+            # no counters: every projection takes the exact map
             "learned_accepted": acc,
-            "learned_fallback": fb,
-            "crushed_inside_surface": ins,
-            "fallback_rate": fb / max(acc + fb, 1),
+            "learned_fallback": fb if branches else None,
+            "crushed_inside_surface": ins if branches else None,
+            "fallback_rate": fb / max(acc + fb, 1) if branches else 1.0,
             "map_branches_network_on": branches,
             "autotune_probe_branch_stats": (dict(zip(("tension_accepted", "tension_fell_back", "shear_accepted",
                                                       "shear_fell_back"), probe_branch_stats))

@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                                                          size_t pitch, const typename Rows<F32>::G *__restrict__ gridV,
                                                          double *__restrict__ gridOut, size_t gpitch,
                                                          int *__restrict__ err, const short *__restrict__ grains,
-                                                         LearnArgs la, MapQueue mq) {
+                                                         LearnArgs la, MapQueue mq, int storeVB) {
     const Rows<F32> S{pos, rest, crushedRow, nullptr, pitch};
     __shared__ __attribute__((aligned(16))) double smem[4][kWaveWinBytes / 8];
     const int per = (nChunks + 7) >> 3;
@@ -565,11 +565,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 __builtin_nontemporal_store((V)F10, at(R + pp * 4, u));
                 __builtin_nontemporal_store((V)(F11 - fe0), at(R + pp * 5, u));
             }
-            // v and Bp: the fused pass uses them at once (a deferred point's
-            // go to the map kernel in its queue entry) and the next G2P
-            // recomputes them before any use; the G2P-only pass that ends a
-            // run stores them
-            if constexpr (!P2G) {
+            // v and Bp: a fused pass uses them at once (a deferred point's go
+            // to the map kernel in its queue entry) and the next G2P
+            // recomputes them before any use; the last pass of a run stores
+            // them (storeVB), for readback, checkpoints and a P2G that
+            // starts afresh
+            if (!P2G || storeVB) {
                 R[pt] = (V)vx;
                 R[pp + pt] = (V)vy;
                 R[pp * 6 + pt] = (V)B00;
@@ -581,8 +582,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             if (crushed != crushed0) *at(S.crushed, u) = crushed;
             if constexpr (F32) {
                 // Fe is what the next substep reads back: the P2G takes it at
-                // its stored precision.  v and Bp stay FP64: a fused pass
-                // does not store them, and the next G2P recomputes them.
+                // its stored precision.  v and Bp stay FP64: the next G2P
+                // recomputes them (what the last pass of a run stores is
+                // read only by a fresh P2G and the readback).
                 F00 = fe0 + (double)(V)(F00 - fe0); F01 = (double)(V)F01;
                 F10 = (double)(V)F10; F11 = fe0 + (double)(V)(F11 - fe0);
             }
@@ -654,9 +656,11 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             const double A00 = B00 - cp.scm * t00, A01 = B01 - cp.scm * t01;
             const double A10 = B10 - cp.scm * t01, A11 = B11 - cp.scm * t11;
             const int lx0 = bx - ox, ly0 = by - oy;
-            // unsigned: a stencil left of or below the origin never lands in the window
-            if (bx + 2 < gx && by + 2 < gy && (unsigned)lx0 + 2 < (unsigned)kWX &&
-                (unsigned)ly0 + 2 < (unsigned)kWY) {
+            // 0 <= lx0 <= kWX - 3 and 0 <= ly0 <= kWY - 3, one unsigned
+            // compare each (the origin is the least base cell of the lanes
+            // that get here, so lx0, ly0 >= 0 already)
+            if (bx + 2 < gx && by + 2 < gy && (unsigned)lx0 < (unsigned)(kWX - 2) &&
+                (unsigned)ly0 < (unsigned)(kWY - 2)) {
                 const int e0 = ly0 * kWS + lx0;
                 // The four lanes of a quad (in band order mostly the points of
                 // one cell) walk the 3 x 3 stencil from four different nodes,
@@ -892,7 +896,8 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
 // block of zeroGrid is cleared only if it took mass in its last P2G
 // (zeroHasMass, recorded when that grid was last updated), and a block of
 // vel32 is rewritten at its massless nodes only if it last held a velocity
-// (velNonzero).  The host sets every flag at the start of a run.
+// (velNonzero).  The host sets every flag when a run starts without a
+// P2G left by the previous one.
 __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restrict__ cases,
                                                      const int2 *__restrict__ nodeBlock,
                                                      double *__restrict__ grid, double *__restrict__ zeroGrid,
@@ -1106,10 +1111,22 @@ struct icempm_batch {
     MapEntry *mapQ = nullptr;          // deferred points (learned map), nPts entries
     unsigned int *mapCount = nullptr;  // two counters, used alternately
 
-    hipGraphExec_t graph = nullptr;
-    int graphSteps = 0;
-    const double *graphT = nullptr;   // the time table and history rows the graph was captured with
-    double *graphH = nullptr;
+    // executable graphs of the substep loop, one per start: fresh (slot 0),
+    // or the first substep's P2G already in grid A (slot 1) or grid B (slot
+    // 2); each with the step count, time table and history rows it was
+    // captured with, and the grid that holds the next P2G after it
+    struct GraphSlot {
+        hipGraphExec_t exec = nullptr;
+        int steps = 0;
+        const double *T = nullptr;
+        double *H = nullptr;
+        int end = -1;
+    };
+    GraphSlot graphs[3];
+    // the grid (0: A, 1: B) that holds the P2G of the next substep, made
+    // from the current points by the last pass of the previous run; -1 when
+    // there is none, or the points have changed since (restore, re-sort)
+    int pending = -1;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float lastMs = 0;
 
@@ -1143,6 +1160,7 @@ struct icempm_batch {
     int timingStride = 0;
     std::vector<hipEvent_t> tev;   // pairs
     hipEvent_t evAfter = nullptr;  // while set: recorded after the next particle kernel
+    bool storeVB = false;          // while set: the next particle kernel stores v and Bp
     size_t tevNext = 0;
     double timedMs = 0;
     int timedLaunches = 0;
@@ -1177,7 +1195,7 @@ void launch_one(icempm_batch *b, unsigned nb, const void *gV, double *gO, const 
     hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream,
                        b->chunks, b->nChunks, b->cases, S.pos, S.rest, S.crushed, S.pitch,
                        static_cast<const typename Rows<F32>::G *>(gV), gO, b->gpitch,
-                       b->err, b->grain, la, mq);
+                       b->err, b->grain, la, mq, b->storeVB ? 1 : 0);
     if (b->evAfter) (void)hipEventRecord(b->evAfter, b->stream);
     if (G2P && RMAP == kMapLearned && la.branches != 0) {
         // the deferred points: map, store, and their P2G into the pass's output grid
@@ -1201,9 +1219,11 @@ void launch_map(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const void *gV
 template <bool F32>
 void launch_prec(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const void *gV, double *gO,
                  const LearnArgs &la, const MapQueue &mq) {
+    // the learned map with every branch switched off is the exact map: its
+    // pass, without the queue logic and the learned map's counters
     if (b->rmap == kMapHarvest)
         launch_map<F32, kMapHarvest>(b, nb, g2p, p2g, gV, gO, la, mq);
-    else if (b->rmap == kMapLearned)
+    else if (b->rmap == kMapLearned && b->mapBranches != 0)
         launch_map<F32, kMapLearned>(b, nb, g2p, p2g, gV, gO, la, mq);
     else
         launch_map<F32, kMapExact>(b, nb, g2p, p2g, gV, gO, la, mq);
@@ -1275,9 +1295,10 @@ void chunk_table(const icempm_batch *b, const std::vector<int> &keys, std::vecto
 }
 
 void drop_graph(icempm_batch *b) {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-    b->graphSteps = 0;
+    for (auto &g : b->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = icempm_batch::GraphSlot{};
+    }
 }
 
 // the host chunk table to the device (grows the device array when needed)
@@ -1378,6 +1399,7 @@ int resort(icempm_batch *b) {
     std::swap(b->grain, b->tgrain);
     std::swap(b->orig, b->torig);
     drop_graph(b);   // the captured launches hold the old row addresses
+    b->pending = -1;   // a P2G in a grid scattered the points from their old slots
     // the sorted keys give the new chunk table
     BCHK(b, hipMemcpyAsync(local.data(), b->sKeys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->stream), "re-sort");
     BCHK(b, hipStreamSynchronize(b->stream), "re-sort");
@@ -1399,53 +1421,65 @@ int rebuild_chunks(icempm_batch *b) {
 }
 
 // The substep loop of n substeps starting from grid A, as stream work.
-hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist) {
-    hipError_t e = launch_fill_f64(b->gridA, b->gpitch * 3, 0.0, b->stream);
-    if (e != hipSuccess) return e;
+// The substep loop: n times the grid update and the fused pass (G2P of
+// this substep, P2G of the next into the other grid).  A fresh start (start
+// < 0) clears grid A and scatters the points into it first; start = 0 or 1
+// takes the P2G the previous run's last pass left in grid A or B, with the
+// block flags that run left.  *end receives the grid that then holds the
+// next substep's P2G.
+hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist, int start, int *end) {
+    hipError_t e;
+    const size_t nblk = (size_t)b->nNodeBlocks;
+    double *A = b->gridA, *B = b->gridB;
+    int *fA = b->blkFlags, *fB = b->blkFlags + nblk, *fV = b->blkFlags + 2 * nblk;
+    if (start < 0 && (e = launch_fill_f64(b->gridA, b->gpitch * 3, 0.0, b->stream)) != hipSuccess) return e;
     if ((e = launch_fill_f64(hist, 2 * (size_t)n * b->hc.size(), 0.0, b->stream)) != hipSuccess) return e;
     if (b->mapCount &&
         (e = launch_fill_i32(reinterpret_cast<int *>(b->mapCount), 2, 0, b->stream)) != hipSuccess)
         return e;
-    if ((e = launch_pass(b, false, true, b->gridA, b->gridA, 0)) != hipSuccess) return e;
-    double *A = b->gridA, *B = b->gridB;
-    const size_t nblk = (size_t)b->nNodeBlocks;
-    int *fA = b->blkFlags, *fB = b->blkFlags + nblk, *fV = b->blkFlags + 2 * nblk;
-    // the grids' and vel32's contents are unknown at the start of a run
-    if ((e = launch_fill_i32(b->blkFlags, 3 * nblk, 1, b->stream)) != hipSuccess) return e;
+    int cur = 0;   // which grid A is
+    if (start < 0) {
+        if ((e = launch_pass(b, false, true, b->gridA, b->gridA, 0)) != hipSuccess) return e;
+        // the grids' and vel32's contents are unknown at a fresh start
+        if ((e = launch_fill_i32(b->blkFlags, 3 * nblk, 1, b->stream)) != hipSuccess) return e;
+    } else if (start == 1) {
+        std::swap(A, B);
+        std::swap(fA, fB);
+        cur = 1;
+    }
     const size_t nc = b->hc.size();
     for (int i = 0; i < n; ++i) {
-        const bool last = (i == n - 1);
         hipLaunchKernelGGL(k_batch_update, dim3(b->nNodeBlocks), dim3(256), 0, b->stream, b->cases,
-                           b->nodeBlock, A, last ? nullptr : B, b->gpitch, tTab, i,
-                           hist + 2 * nc * (size_t)i, b->vel32, fA, fB, fV, b->bins);
+                           b->nodeBlock, A, B, b->gpitch, tTab, i, hist + 2 * nc * (size_t)i, b->vel32, fA, fB,
+                           fV, b->bins);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         const void *gV = b->vel32 ? static_cast<const void *>(b->vel32) : static_cast<const void *>(A);
-        if (!last) {
-            const bool timeIt = b->timingStride > 0 && i % b->timingStride == 0;
-            hipEvent_t t0 = nullptr, t1 = nullptr;
-            if (timeIt) {
-                if (b->tevNext + 2 > b->tev.size()) {
-                    for (int q = 0; q < 2; ++q) {
-                        hipEvent_t ev;
-                        if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
-                        b->tev.push_back(ev);
-                    }
+        const bool timeIt = b->timingStride > 0 && i % b->timingStride == 0;
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (timeIt) {
+            if (b->tevNext + 2 > b->tev.size()) {
+                for (int q = 0; q < 2; ++q) {
+                    hipEvent_t ev;
+                    if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+                    b->tev.push_back(ev);
                 }
-                t0 = b->tev[b->tevNext];
-                t1 = b->tev[b->tevNext + 1];
-                b->tevNext += 2;
-                if ((e = hipEventRecord(t0, b->stream)) != hipSuccess) return e;
             }
-            b->evAfter = timeIt ? t1 : nullptr;   // recorded right after the particle kernel
-            e = launch_pass(b, true, true, gV, B, i);
-            b->evAfter = nullptr;
-            if (e != hipSuccess) return e;
-            std::swap(A, B);
-            std::swap(fA, fB);
-        } else {
-            if ((e = launch_pass(b, true, false, gV, nullptr, i)) != hipSuccess) return e;
+            t0 = b->tev[b->tevNext];
+            t1 = b->tev[b->tevNext + 1];
+            b->tevNext += 2;
+            if ((e = hipEventRecord(t0, b->stream)) != hipSuccess) return e;
         }
+        b->evAfter = timeIt ? t1 : nullptr;   // recorded right after the particle kernel
+        b->storeVB = (i == n - 1);
+        e = launch_pass(b, true, true, gV, B, i);
+        b->evAfter = nullptr;
+        b->storeVB = false;
+        if (e != hipSuccess) return e;
+        std::swap(A, B);
+        std::swap(fA, fB);
+        cur ^= 1;
     }
+    *end = cur;
     return hipSuccess;
 }
 
@@ -1473,7 +1507,7 @@ int icempm_batch_create(icempm_batch **out, int precision) {
 void icempm_batch_destroy(icempm_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    drop_graph(b);
     dfree(b->pos); dfree(b->rest); dfree(b->crushed); dfree(b->gridA); dfree(b->gridB); dfree(b->vel32);
     dfree(b->blkFlags);
     dfree(b->cases); dfree(b->chunks); dfree(b->nodeBlock);
@@ -1699,40 +1733,52 @@ int icempm_batch_finalize(icempm_batch *b) {
 
 namespace {
 
-// n substeps from grid A with the given time table and force-history rows,
-// as one graph launch (captured when the step count or the pointers change)
+// n substeps with the given time table and force-history rows, from the
+// P2G the previous run left (b->pending) or from a fresh P2G, as one graph
+// launch (captured when the start, the step count or the pointers change)
 // or as stream launches; returns the device time in ms.
 int run_segment(icempm_batch *b, int n, int use_graph, const double *tTab, double *hist, float *ms) {
     b->tevNext = 0;
+    const int start = b->pending;
+    b->pending = -1;   // until the loop below is enqueued whole
+    int end = -1;
     if (use_graph) {
-        if (!b->graph || b->graphSteps != n || b->graphT != tTab || b->graphH != hist) {
-            drop_graph(b);
-            hipGraph_t g = nullptr;
+        icempm_batch::GraphSlot &g = b->graphs[start + 1];
+        if (!g.exec || g.steps != n || g.T != tTab || g.H != hist) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = icempm_batch::GraphSlot{};
+            hipGraph_t graph = nullptr;
             BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch capture");
-            hipError_t e = enqueue_loop(b, n, tTab, hist);
-            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-            if (e != hipSuccess) return bfail(b, ICEMPM_E_HIP, std::string("batch capture: ") + hipGetErrorString(e));
+            hipError_t e = enqueue_loop(b, n, tTab, hist, start, &end);
+            hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+            if (e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return bfail(b, ICEMPM_E_HIP, std::string("batch capture: ") + hipGetErrorString(e));
+            }
             BCHK(b, e2, "batch capture");
-            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
+            e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
             BCHK(b, e, "batch graph instantiate");
-            b->graphSteps = n;
-            b->graphT = tTab;
-            b->graphH = hist;
+            g.steps = n;
+            g.T = tTab;
+            g.H = hist;
+            g.end = end;
             // put the executable graph on the device now, so that its first
             // launch (timed below) costs what a replay costs
-            BCHK(b, hipGraphUpload(b->graph, b->stream), "batch graph upload");
+            BCHK(b, hipGraphUpload(g.exec, b->stream), "batch graph upload");
             BCHK(b, hipStreamSynchronize(b->stream), "batch graph upload");
         }
+        end = g.end;
         // timed from here: capture and instantiation are host work before it
         BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
-        BCHK(b, hipGraphLaunch(b->graph, b->stream), "batch graph launch");
+        BCHK(b, hipGraphLaunch(g.exec, b->stream), "batch graph launch");
     } else {
         BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
-        BCHK(b, enqueue_loop(b, n, tTab, hist), "batch run");
+        BCHK(b, enqueue_loop(b, n, tTab, hist, start, &end), "batch run");
     }
     BCHK(b, hipEventRecord(b->e1, b->stream), "batch run");
     BCHK(b, hipEventSynchronize(b->e1), "batch run");
+    b->pending = end;
     BCHK(b, hipEventElapsedTime(ms, b->e0, b->e1), "batch run");
     for (size_t q = 0; q + 1 < b->tevNext; q += 2) {
         float kms = 0;
@@ -1930,11 +1976,7 @@ int icempm_batch_set_return_map(icempm_batch *b, int mode, const float *params, 
         return bfail(b, ICEMPM_E_ARG, "set_return_map: tolerance is NaN");
     BCHK(b, hipStreamSynchronize(b->stream), "set_return_map");
     // a captured loop holds the previous kernel arguments
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-        b->graphSteps = 0;
-    }
+    drop_graph(b);
     if (mode == kMapHarvest) {
         if (harvest_capacity != b->hcap) {
             dfree(b->hx);
@@ -2068,6 +2110,7 @@ int icempm_batch_restore(icempm_batch *b) {
         BCHK(b, hipMemcpyAsync(b->grain, b->kgrain, b->pitch * sizeof(short), hipMemcpyDeviceToDevice, b->stream),
              "restore");
     BCHK(b, hipStreamSynchronize(b->stream), "restore");
+    b->pending = -1;   // the grids hold the P2G of the state before the restore
     b->t = b->kt;
     b->step = b->kstep;
     b->hcstart = b->kcstart;

@@ -327,7 +327,8 @@ int icempm_batch_map_stats(icempm_batch *b, unsigned long long *out, int reset);
 int icempm_batch_map_branch_stats(icempm_batch *b, unsigned long long *out, int reset);
 /* Branches the network proposes for (ICEMPM_MAP_TENSION | ICEMPM_MAP_SHEAR,
  * both by default); a switched-off branch takes the exact map in the particle
- * pass, with no queue, and with both off no map kernel runs.  In learned mode
+ * pass, with no queue, and with both off the pass is the exact map's (no map
+ * kernel, and the learned map's counters stay as they are).  In learned mode
  * icempm_batch_autotune switches off each branch whose proposals passed the
  * yield gate less often than ICEMPM_BATCH_TUNE_MAP_MIN_ACCEPT percent over
  * its probes; autotune_branch_stats returns those probes' counts (as

@@ -76,6 +76,66 @@ def test_batch_graph_replay_matches_stream_run(gpu_available):
     s.close()
 
 
+@pytest.mark.parametrize("use_graph", [True, False], ids=["graph", "stream"])
+def test_runs_continue_from_the_previous_runs_p2g(gpu_available, use_graph):
+    """A run's last pass leaves the next substep's P2G in one of the two
+    grids and the next run starts from it: runs of 7, 13, 1 and 9 substeps
+    (odd lengths alternate the grid that holds it) follow the oracle's
+    30-substep run as one run does, force history included; a restore, or a
+    re-sort between runs, makes the next run start with a P2G of its own."""
+    print("batch: runs continue from the previous P2G", use_graph, flush=True)
+    cases = batch_cases(3, 1500, 4000, seed=14)
+    b = build_batch(cases, BATCH_F64)
+    lens = [7, 13, 1, 9]
+    nsub = sum(lens)
+    refs = [oracle_case(*b.initial[c], nsub) for c in range(len(cases))]
+    b.checkpoint()
+
+    def go(label, between=None):
+        hist = [[] for _ in cases]
+        for k, steps in enumerate(lens):
+            if between is not None and k == 2:
+                between()
+            b.run(steps, use_graph=use_graph)
+            for c in range(len(cases)):
+                hist[c].append(b.force_history(c, steps))
+        assert b.error_flag() == 0
+        for c in range(len(cases)):
+            assert rel_l2(np.concatenate(hist[c]), refs[c][0]) <= 1e-6, (label, c)
+        check_state(label)
+
+    def check_state(label):
+        """positions to 1e-12 m; velocities, Fe and Bp (which a run's last
+        pass stores for readback) to 1e-8 of each row's largest value"""
+        for c, case in enumerate(cases):
+            n = case.n_points
+            pref = refs[c][1]
+            dev = b.points(c)
+            assert np.abs(dev[4:6, :n] - pref[4:6, :n]).max() <= 1e-12, (label, c)
+            for r in (6, 7, 8, 9, 10, 11, 12, 13, 14, 15):
+                scale = np.abs(pref[r, :n]).max()
+                assert np.abs(dev[r, :n] - pref[r, :n]).max() <= 1e-8 * scale, (label, c, r)
+
+    go("continued")
+    b.restore()
+    go("after a restore")
+    b.restore()
+    go("re-sorted between runs", between=b.resort)
+    # a checkpoint taken after a run holds its velocities and Bp
+    b.restore()
+    b.run(10, use_graph=use_graph)
+    b.checkpoint()
+    b.run(20, use_graph=use_graph)
+    h1 = [b.force_history(c, 20) for c in range(len(cases))]
+    b.restore()
+    b.run(20, use_graph=use_graph)
+    for c in range(len(cases)):
+        assert rel_l2(b.force_history(c, 20), refs[c][0][10:]) <= 1e-6, c
+        assert rel_l2(h1[c], refs[c][0][10:]) <= 1e-6, c
+    check_state("restored from a checkpoint taken after a run")
+    b.close()
+
+
 def test_resort_chunk_sizes_and_checkpoint_match_oracle(gpu_available):
     """Re-sorting between segments, other points-per-chunk values and a
     checkpoint/restore leave every case on the oracle's trajectory, and the

@@ -164,9 +164,10 @@ def test_trained_map_force_error_and_gain(gpu_available):
     # The gate decides where the network runs at all: the autotuner counts,
     # per branch of the map, how often the network's proposals passed the
     # yield gate over its probes, and switches a branch below 50% off (its
-    # projections then take the exact map in the particle pass: no queue, no
-    # map kernel).  At 1e-4 both branches go off and the run is the exact
-    # map's; at 1e-2 the network keeps the branches it serves.
+    # projections then take the exact map in the particle pass: no queue; with
+    # both off the pass is the exact map's).  At 1e-4 both branches go off and
+    # the run is the exact map's; at 1e-2 the network keeps the branches it
+    # serves.
     for tol in (1e-4, 1e-2):
         b = build_batch(cs, BATCH_F64)
         b.set_learned(params, norm, tol=tol)
