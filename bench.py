@@ -54,10 +54,10 @@ BATCH_F64_BYTES_PER_POINT = 212
 # writes them; Jp_inv is read only by crushed points, and it and the crushed
 # flag are written only when they change.  FP64 single case: read pos 16 +
 # Fe 32 + crushed 1 + grain 2, write pos 16 + Fe 32 = 99 B; batched FP32
-# storage (positions 2 x 32-bit fixed point, no grain row): read 8 + 16 + 1,
-# write 8 + 16 = 49 B.
+# storage (positions 48-bit fixed point in three 32-bit rows, no grain row):
+# read 12 + 16 + 1, write 12 + 16 = 57 B.
 FUSED_MOVED_BYTES_PER_POINT = 99
-BATCH_F32_MOVED_BYTES_PER_POINT = 49
+BATCH_F32_MOVED_BYTES_PER_POINT = 57
 # With FP32 point storage the batched gathers read each node's velocity as
 # an FP32 pair: 24 B written by the P2G + 8 B read per node.
 BATCH_F32_MOVED_BYTES_PER_NODE = 32
@@ -345,6 +345,8 @@ def run_batch_learned(args) -> dict:
         tune_rows = b.autotune(args.autotune_steps)
         log(f"bench: autotune {time.perf_counter() - t0:.1f} s -> chunk {b.tuning()[0]}, "
             f"re-sort every {b.tuning()[1]}: " + ", ".join(f"({c},{r}) {t * 1e3:.0f} us" for c, r, t in tune_rows))
+    if args.map_branches is not None:
+        b.set_map_branches(args.map_branches)
     chunk, resort_every = b.tuning()
     branches = b.map_branches()
     probe_branch_stats = b.autotune_branch_stats() if args.autotune_steps > 0 else None
@@ -409,7 +411,9 @@ def run_batch_learned(args) -> dict:
         log(f"bench: CPU baseline (~{args.cpu_seconds:.0f} s)")
         cpu = cpu_baseline(states, args.cpu_seconds, f"the {len(cases)} cases of this workload")
 
-    traffic = traffic_from_profiles("batch_particles_learned_32cases")
+    # the committed counter summary of the kernel the timed run launched
+    prof = f"batch_particles_f32_{'learned' if branches else 'exact'}_32cases"
+    traffic = traffic_from_profiles(prof)
     return {
         "metric": METRIC,
         "value": value,
@@ -449,7 +453,8 @@ def run_batch_learned(args) -> dict:
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic,
             "hbm_frac_counters": (traffic / k_avg_s / 1e9 / HBM_PEAK_GBS) if (traffic and k_avg_s) else None,
-            "atomics": atomics_from_profiles("batch_particles_learned_32cases", k_avg_s),
+            "atomics": atomics_from_profiles(prof, k_avg_s),
+            "counter_profile": f"profiles/{prof}_counters.txt",
             "bytes_per_launch": k_bytes,
             "moved_bytes_per_launch": k_moved,
             "moved_gbs": (k_moved / k_avg_s / 1e9) if k_avg_s else None,
@@ -514,6 +519,9 @@ def main() -> int:
                     help="batch: fewest timed substeps per case of the plain single-case comparison")
     ap.add_argument("--autotune-steps", type=int, default=200,
                     help="batch: substeps per autotuner candidate (0: keep the defaults)")
+    ap.add_argument("--map-branches", type=int, choices=[0, 1, 2, 3], default=None,
+                    help="batch: branches the network proposes for, 1 tension + 2 shear "
+                         "(default: as the autotuner leaves them)")
     ap.add_argument("--kernel-steps", type=int, default=200,
                     help="batch: stream-launched substeps for the fused-pass timing")
     ap.add_argument("--timing-stride", type=int, default=4,
