@@ -142,7 +142,7 @@ HIP_EXPORTS = [
     "icempm_batch_checkpoint", "icempm_batch_restore", "icempm_batch_autotune",
     "icempm_mlp_num_params", "icempm_mlp_create", "icempm_mlp_destroy", "icempm_mlp_last_error",
     "icempm_mlp_get_params", "icempm_mlp_device_params", "icempm_mlp_step", "icempm_mlp_infer",
-    "icempm_mfma_probe", "icempm_device_alloc", "icempm_device_free", "icempm_copy_to_device",
+    "icempm_mfma_probe", "icempm_mfma_probe_k", "icempm_device_alloc", "icempm_device_free", "icempm_copy_to_device",
     "icempm_copy_to_host",
 ]
 
@@ -254,6 +254,7 @@ def hip():
     _sig(L, "icempm_mlp_step", c_int, P, c_void_p, c_void_p, c_int, c_int, c_float, fp, fp)
     _sig(L, "icempm_mlp_infer", c_int, P, c_void_p, c_int, c_void_p)
     _sig(L, "icempm_mfma_probe", c_int, fp, fp, fp)
+    _sig(L, "icempm_mfma_probe_k", c_int, fp, fp, fp, c_int)
     _sig(L, "icempm_device_alloc", c_int, c_size_t, POINTER(c_void_p))
     _sig(L, "icempm_device_free", None, c_void_p)
     _sig(L, "icempm_copy_to_device", c_int, c_void_p, c_void_p, c_size_t)

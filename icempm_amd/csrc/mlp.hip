@@ -1,6 +1,7 @@
 // mlp.hip -- the learned return map: a 5 -> 64 -> 64 -> 3 MLP with tanh
 // hidden layers, BF16 weights and activations on the MFMA units
-// (v_mfma_f32_16x16x16_bf16, FP32 accumulation), trained on the GPU by the
+// (v_mfma_f32_16x16x32_bf16 for the 64-long products, v_mfma_f32_16x16x16_bf16
+// for the 16-long ones, FP32 accumulation), trained on the GPU by the
 // kernels below (forward, backward, Adam) and evaluated by k_mlp_infer and by
 // the batch particle pass.
 //
@@ -13,9 +14,10 @@
 //   b3[16]                                                  offset 6272
 // W[out][in]: y = W x + b.  Inputs and outputs are normalised by the caller.
 //
-// MFMA 16x16x16 lane layout (wavefront of 64): lane l holds A[l%16][4g..4g+3]
-// and B[4g..4g+3][l%16] with g = l/16, and receives D[4g+i][l%16], i < 4.
-// tests/test_gpu_mlp.py checks this layout against a CPU product.
+// MFMA lane layout (wavefront of 64, g = l/16): a 32-long step gives lane l
+// A[l%16][8g..8g+7] and B[8g..8g+7][l%16], a 16-long step A[l%16][4g..4g+3]
+// and B[4g..4g+3][l%16]; lane l receives D[4g+i][l%16], i < 4.
+// tests/test_gpu_mlp.py checks both layouts against a CPU product.
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -35,6 +37,18 @@ using namespace mlpdev;
 template <class FA, class FB>
 static __device__ __forceinline__ f4 tile(FA a, FB b, int K, f4 acc) {
     const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+    if (K % 32 == 0) {
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            s8 av, bv;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                av[j] = (short)to_bf16(a(r, k0 + 8 * g + j));
+                bv[j] = (short)to_bf16(b(k0 + 8 * g + j, r));
+            }
+            acc = mfma32(av, bv, acc);
+        }
+        return acc;
+    }
     for (int k0 = 0; k0 < K; k0 += 16) {
         s4 av, bv;
 #pragma unroll
@@ -47,12 +61,14 @@ static __device__ __forceinline__ f4 tile(FA a, FB b, int K, f4 acc) {
     return acc;
 }
 
-// Lane layout probe: D = A * B for one 16x16x16 tile, A and B row-major.
-__global__ void k_mfma_probe(const float *A, const float *B, float *D) {
+// Lane layout probe: D = A * B, A (16 x K) and B (K x 16) row-major, by the
+// tile routine of the training kernels (K = 16: the 16x16x16 form; K a
+// multiple of 32: the 16x16x32 form).
+__global__ void k_mfma_probe(const float *A, const float *B, float *D, int K) {
     const int l = threadIdx.x, g = l >> 4, c = l & 15;
     f4 acc = {0, 0, 0, 0};
-    acc = tile([&](int m, int k) { return A[m * 16 + k]; }, [&](int k, int n) { return B[k * 16 + n]; },
-               16, acc);
+    acc = tile([&](int m, int k) { return A[m * K + k]; }, [&](int k, int n) { return B[k * 16 + n]; },
+               K, acc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + c] = acc[i];
 }
@@ -456,20 +472,24 @@ int icempm_copy_to_host(void *dst, const void *src, size_t nbytes) {
     return hipMemcpy(dst, src, nbytes, hipMemcpyDeviceToHost) == hipSuccess ? ICEMPM_OK : ICEMPM_E_HIP;
 }
 
-// D (16 x 16, host) = A (16 x 16) * B (16 x 16) through one MFMA tile, for
-// the lane-layout test.
-int icempm_mfma_probe(const float *A, const float *B, float *D) {
+// D (16 x 16, host) = A (16 x K) * B (K x 16) through the MFMA tile routine,
+// for the lane-layout test; K is 16, 32 or 64.
+int icempm_mfma_probe_k(const float *A, const float *B, float *D, int K) {
+    if (!A || !B || !D || (K != 16 && K != 32 && K != 64)) return ICEMPM_E_ARG;
+    const size_t na = 16 * (size_t)K, nb = (size_t)K * 16;
     float *d = nullptr;
-    hipError_t e = hipMalloc(&d, 3 * 256 * sizeof(float));
-    if (e == hipSuccess) e = hipMemcpy(d, A, 256 * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d + 256, B, 256 * sizeof(float), hipMemcpyHostToDevice);
+    hipError_t e = hipMalloc(&d, (na + nb + 256) * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(d, A, na * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + na, B, nb * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, d, d + 256, d + 512);
+        hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, d, d + na, d + na + nb, K);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpy(D, d + 512, 256 * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(D, d + na + nb, 256 * sizeof(float), hipMemcpyDeviceToHost);
     if (d) (void)hipFree(d);
     return e == hipSuccess ? ICEMPM_OK : ICEMPM_E_HIP;
 }
+
+int icempm_mfma_probe(const float *A, const float *B, float *D) { return icempm_mfma_probe_k(A, B, D, 16); }
 
 }  // extern "C"

@@ -117,12 +117,16 @@ class MLP:
 
 
 def mfma_probe(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """D = A (16 x K) B (K x 16) through the MFMA tile routine, K in {16, 32, 64}."""
     H = _lib.hip()
     A = np.ascontiguousarray(A, dtype=np.float32)
     B = np.ascontiguousarray(B, dtype=np.float32)
+    if A.shape[0] != 16 or B.shape[1] != 16 or A.shape[1] != B.shape[0]:
+        raise ValueError("mfma_probe: A must be 16 x K and B K x 16")
     D = np.zeros((16, 16), dtype=np.float32)
     fp = POINTER(c_float)
-    rc = H.icempm_mfma_probe(A.ctypes.data_as(fp), B.ctypes.data_as(fp), D.ctypes.data_as(fp))
+    rc = H.icempm_mfma_probe_k(A.ctypes.data_as(fp), B.ctypes.data_as(fp), D.ctypes.data_as(fp),
+                               int(A.shape[1]))
     if rc != 0:
         raise IcempmError(rc, "mfma_probe failed")
     return D

@@ -364,6 +364,10 @@ int icempm_copy_to_device(void *dst, const void *src, size_t nbytes);
 int icempm_copy_to_host(void *dst, const void *src, size_t nbytes);
 /* one 16x16x16 BF16 MFMA tile on host matrices (row-major), for layout tests */
 int icempm_mfma_probe(const float *A, const float *B, float *D);
+/* D (16 x 16) = A (16 x K) * B (K x 16), row-major host matrices, through the
+ * training kernels' tile routine: K = 16 the 16x16x16 form, K = 32 or 64 the
+ * 16x16x32 form (one or two steps); for layout tests */
+int icempm_mfma_probe_k(const float *A, const float *B, float *D, int K);
 
 #ifdef __cplusplus
 }

@@ -9,11 +9,15 @@ from icempm_amd.learned import MLP, DeviceArray, init_params, mfma_probe
 pytestmark = pytest.mark.gpu
 
 
-def test_mfma_tile_layout(gpu_available):
-    print("mfma probe", flush=True)
-    rng = np.random.default_rng(1)
-    A = rng.integers(-8, 8, size=(16, 16)).astype(np.float32)   # exact in bf16
-    B = rng.integers(-8, 8, size=(16, 16)).astype(np.float32)
+@pytest.mark.parametrize("K", [16, 32, 64])
+def test_mfma_tile_layout(gpu_available, K):
+    """The tile routine's lane layout: 16x16x16 steps for K = 16, gfx950's
+    16x16x32 form for K = 32 and 64 (small integers: every product and sum is
+    exact in BF16 operands with FP32 accumulation)."""
+    print("mfma probe", K, flush=True)
+    rng = np.random.default_rng(K)
+    A = rng.integers(-8, 8, size=(16, K)).astype(np.float32)
+    B = rng.integers(-8, 8, size=(K, 16)).astype(np.float32)
     D = mfma_probe(A, B)
     assert np.array_equal(D, A @ B)
 
