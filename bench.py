@@ -228,6 +228,7 @@ def run_single(args) -> dict:
         log(f"bench: CPU baseline (~{args.cpu_seconds:.0f} s)")
         cpu = cpu_baseline([(prms0, pts0)], args.cpu_seconds, case.name)
 
+    traffic = traffic_from_profiles(f"fused_g2p_p2g_{case.name}")
     return {
         "metric": METRIC,
         "value": value,
@@ -257,8 +258,11 @@ def run_single(args) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": traffic_from_profiles(f"fused_g2p_p2g_{case.name}"),
+            "traffic": traffic,
+            # the committed counters' HBM bytes over this run's launch time
+            "hbm_frac_counters": (traffic / k_avg_s / 1e9 / HBM_PEAK_GBS) if (traffic and k_avg_s) else None,
             "atomics": atomics_from_profiles(f"fused_g2p_p2g_{case.name}", k_avg_s),
+            "counter_profile": f"profiles/fused_g2p_p2g_{case.name}_counters.txt",
             "bytes_per_launch": k_bytes,
             "moved_bytes_per_launch": k_moved,
             "moved_gbs": (k_moved / k_avg_s / 1e9) if k_avg_s else None,

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int16, c_long, c_size_t,
+from ctypes import (POINTER, c_ulonglong, c_char_p, c_double, c_float, c_int, c_int16, c_long, c_size_t,
                     c_uint64, c_void_p)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -142,7 +142,7 @@ HIP_EXPORTS = [
     "icempm_batch_checkpoint", "icempm_batch_restore", "icempm_batch_autotune",
     "icempm_mlp_num_params", "icempm_mlp_create", "icempm_mlp_destroy", "icempm_mlp_last_error",
     "icempm_mlp_get_params", "icempm_mlp_device_params", "icempm_mlp_step", "icempm_mlp_infer",
-    "icempm_mfma_probe", "icempm_mfma_probe_k", "icempm_device_alloc", "icempm_device_free", "icempm_copy_to_device",
+    "icempm_mfma_probe", "icempm_mfma_probe_k", "icempm_mlp_train", "icempm_device_alloc", "icempm_device_free", "icempm_copy_to_device",
     "icempm_copy_to_host",
 ]
 
@@ -255,6 +255,7 @@ def hip():
     _sig(L, "icempm_mlp_infer", c_int, P, c_void_p, c_int, c_void_p)
     _sig(L, "icempm_mfma_probe", c_int, fp, fp, fp)
     _sig(L, "icempm_mfma_probe_k", c_int, fp, fp, fp, c_int)
+    _sig(L, "icempm_mlp_train", c_int, fp, fp, c_int, c_int, c_int, c_float, c_ulonglong, fp, fp, fp, c_int, fp)
     _sig(L, "icempm_device_alloc", c_int, c_size_t, POINTER(c_void_p))
     _sig(L, "icempm_device_free", None, c_void_p)
     _sig(L, "icempm_copy_to_device", c_int, c_void_p, c_void_p, c_size_t)

@@ -472,6 +472,124 @@ int icempm_copy_to_host(void *dst, const void *src, size_t nbytes) {
     return hipMemcpy(dst, src, nbytes, hipMemcpyDeviceToHost) == hipSuccess ? ICEMPM_OK : ICEMPM_E_HIP;
 }
 
+// The training driver (what icempm_amd/learned.py's train_return_map did in
+// Python): normalisation fitted on the pairs, one shuffle of the rows, Glorot
+// initial weights, `steps` Adam steps on consecutive windows of `batch` rows
+// with a cosine learning-rate decay to lr / 20, and the loss over all rows.
+namespace {
+
+unsigned long long splitmix64(unsigned long long &s) {
+    unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double uniform01(unsigned long long &s) { return ((splitmix64(s) >> 11) + 0.5) * 0x1.0p-53; }
+// a standard normal variate (Box-Muller)
+double normal01(unsigned long long &s) {
+    const double u = uniform01(s), v = uniform01(s);
+    return std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * v);
+}
+
+}  // namespace
+
+int icempm_mlp_train(const float *X, const float *T, int n, int steps, int batch, float lr,
+                     unsigned long long seed, float *params_out, float *norm_out, float *trace,
+                     int trace_every, float *final_loss) {
+    if (!X || !T || n <= 0 || steps < 0 || batch <= 0 || !(lr >= 0.f) || !params_out || !norm_out ||
+        (trace && trace_every <= 0))
+        return ICEMPM_E_ARG;
+    // normalisation: mean and (population) standard deviation per column;
+    // a constant input column (the hardening feature) gets mean 0, scale 1
+    double im[kRealIn] = {}, is[kRealIn] = {}, om[kRealOut] = {}, os[kRealOut] = {};
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < kRealIn; ++k) im[k] += X[(size_t)i * kRealIn + k];
+        for (int k = 0; k < kRealOut; ++k) om[k] += T[(size_t)i * kRealOut + k];
+    }
+    for (int k = 0; k < kRealIn; ++k) im[k] /= n;
+    for (int k = 0; k < kRealOut; ++k) om[k] /= n;
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < kRealIn; ++k) {
+            const double d = X[(size_t)i * kRealIn + k] - im[k];
+            is[k] += d * d;
+        }
+        for (int k = 0; k < kRealOut; ++k) {
+            const double d = T[(size_t)i * kRealOut + k] - om[k];
+            os[k] += d * d;
+        }
+    }
+    for (int k = 0; k < kRealIn; ++k) {
+        is[k] = std::sqrt(is[k] / n);
+        if (is[k] < 1e-12) {
+            im[k] = 0.0;
+            is[k] = 1.0;
+        }
+    }
+    for (int k = 0; k < kRealOut; ++k) {
+        os[k] = std::sqrt(os[k] / n);
+        if (os[k] < 1e-30) os[k] = 1.0;
+    }
+    for (int k = 0; k < kRealIn; ++k) {
+        norm_out[k] = (float)im[k];
+        norm_out[kRealIn + k] = (float)is[k];
+    }
+    for (int k = 0; k < kRealOut; ++k) {
+        norm_out[2 * kRealIn + k] = (float)om[k];
+        norm_out[2 * kRealIn + kRealOut + k] = (float)os[k];
+    }
+    // the rows shuffled once (Fisher-Yates), normalised
+    unsigned long long rs = seed;
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; ++i) order[(size_t)i] = i;
+    for (int i = n - 1; i > 0; --i) {
+        const int j = (int)(splitmix64(rs) % (unsigned long long)(i + 1));
+        std::swap(order[(size_t)i], order[(size_t)j]);
+    }
+    std::vector<float> Xn((size_t)n * kRealIn), Tn((size_t)n * kRealOut);
+    for (int r = 0; r < n; ++r) {
+        const size_t i = (size_t)order[(size_t)r];
+        for (int k = 0; k < kRealIn; ++k) Xn[(size_t)r * kRealIn + k] = (float)((X[i * kRealIn + k] - im[k]) / is[k]);
+        for (int k = 0; k < kRealOut; ++k)
+            Tn[(size_t)r * kRealOut + k] = (float)((T[i * kRealOut + k] - om[k]) / os[k]);
+    }
+    // Glorot-scaled normal initial weights, zero biases and padding
+    std::vector<float> P(kParams, 0.f);
+    for (int o = 0; o < kH; ++o)
+        for (int k = 0; k < kRealIn; ++k) P[oW1 + o * kIn + k] = (float)(normal01(rs) * std::sqrt(2.0 / (kRealIn + kH)));
+    for (int e = 0; e < kH * kH; ++e) P[oW2 + e] = (float)(normal01(rs) * std::sqrt(2.0 / (2 * kH)));
+    for (int o = 0; o < kRealOut; ++o)
+        for (int k = 0; k < kH; ++k) P[oW3 + o * kH + k] = (float)(normal01(rs) * std::sqrt(2.0 / (kRealOut + kH)));
+    float *dX = nullptr, *dT = nullptr;
+    icempm_mlp *m = nullptr;
+    int rc = ICEMPM_OK;
+    if (icempm_device_alloc(Xn.size() * sizeof(float), reinterpret_cast<void **>(&dX)) != ICEMPM_OK ||
+        icempm_device_alloc(Tn.size() * sizeof(float), reinterpret_cast<void **>(&dT)) != ICEMPM_OK ||
+        icempm_copy_to_device(dX, Xn.data(), Xn.size() * sizeof(float)) != ICEMPM_OK ||
+        icempm_copy_to_device(dT, Tn.data(), Tn.size() * sizeof(float)) != ICEMPM_OK)
+        rc = ICEMPM_E_HIP;
+    if (rc == ICEMPM_OK) rc = icempm_mlp_create(&m, P.data());
+    if (rc == ICEMPM_OK) {
+        const int B = batch < n ? batch : n;
+        const int windows = n / B > 1 ? n / B : 1;
+        for (int it = 0; it < steps && rc == ICEMPM_OK; ++it) {
+            const int r0 = (it % windows) * B;
+            const double a =
+                (double)lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * it / (steps > 0 ? steps : 1))));
+            float l = 0;
+            rc = icempm_mlp_step(m, dX, dT, r0, B, (float)a, nullptr, &l);
+            if (rc == ICEMPM_OK && trace && it % trace_every == 0) trace[it / trace_every] = l;
+        }
+        float full = 0;
+        if (rc == ICEMPM_OK) rc = icempm_mlp_step(m, dX, dT, 0, n, 0.f, nullptr, &full);
+        if (rc == ICEMPM_OK && final_loss) *final_loss = full;
+        if (rc == ICEMPM_OK) rc = icempm_mlp_get_params(m, params_out);
+    }
+    if (m) icempm_mlp_destroy(m);
+    icempm_device_free(dX);
+    icempm_device_free(dT);
+    return rc;
+}
+
 // D (16 x 16, host) = A (16 x K) * B (K x 16) through the MFMA tile routine,
 // for the lane-layout test; K is 16, 32 or 64.
 int icempm_mfma_probe_k(const float *A, const float *B, float *D, int K) {

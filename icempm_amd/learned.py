@@ -133,57 +133,31 @@ def mfma_probe(A: np.ndarray, B: np.ndarray) -> np.ndarray:
 
 
 # -- training the return map on harvested pairs --------------------------------
-def fit_norm(X: np.ndarray, T: np.ndarray) -> np.ndarray:
-    """Normalisation of features and targets: 16 floats = input mean (5),
-    input scale (5), output mean (3), output scale (3).  Constant columns
-    (the hardening feature) get mean 0 and scale 1."""
-    X = np.asarray(X, dtype=np.float64)
-    T = np.asarray(T, dtype=np.float64)
-    im, isd = X.mean(0), X.std(0)
-    const = isd < 1e-12
-    im[const] = 0.0
-    isd[const] = 1.0
-    om, osd = T.mean(0), T.std(0)
-    osd[osd < 1e-30] = 1.0
-    return np.concatenate([im, isd, om, osd]).astype(np.float32)
-
-
-def normalise(X: np.ndarray, T: np.ndarray, norm: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
-    n = np.asarray(norm, dtype=np.float64)
-    Xn = (np.asarray(X, np.float64) - n[0:5]) / n[5:10]
-    Tn = (np.asarray(T, np.float64) - n[10:13]) / n[13:16]
-    return Xn.astype(np.float32), Tn.astype(np.float32)
-
-
 def train_return_map(X: np.ndarray, T: np.ndarray, steps: int = 3000, batch: int = 8192,
                      lr: float = 3e-3, seed: int = 0, log_every: int = 0):
-    """Train the MLP on the GPU (icempm_mlp_step: MFMA forward and backward,
-    Adam) on harvested pairs.  Rows are shuffled once on the host; each step
-    takes the next contiguous window of ``batch`` rows.  The learning rate
-    follows a cosine decay to lr/20.  Returns (params, norm, losses)."""
-    norm = fit_norm(X, T)
-    Xn, Tn = normalise(X, T, norm)
-    rng = np.random.default_rng(seed)
-    order = rng.permutation(Xn.shape[0])
-    Xn, Tn = Xn[order], Tn[order]
-    n = Xn.shape[0]
-    batch = min(batch, n)
-    dX, dT = DeviceArray(Xn), DeviceArray(Tn)
-    m = MLP(init_params(seed))
-    losses = []
-    try:
-        windows = max(1, n // batch)
-        for it in range(steps):
-            r0 = (it % windows) * batch
-            a = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * it / max(steps, 1))))
-            loss, _ = m.step(dX, dT, r0, batch, float(a))
-            if log_every and it % log_every == 0:
-                losses.append((it, loss))
-        full, _ = m.step(dX, dT, 0, n, 0.0)
-        losses.append((steps, full))
-        params = m.params()
-    finally:
-        m.close()
-        dX.free()
-        dT.free()
+    """Train the MLP on harvested pairs with the library's training driver
+    (icempm_mlp_train: normalisation, one seeded shuffle, Glorot initial
+    weights, Adam on MFMA over consecutive windows of ``batch`` rows, cosine
+    decay of the learning rate to lr/20).  Returns (params, norm, losses):
+    losses holds (step, window loss) every ``log_every`` steps and
+    (steps, loss over all rows) last."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    n = int(X.shape[0])
+    if X.shape != (n, 5) or T.shape != (n, 3):
+        raise ValueError("train_return_map: X must be n x 5 and T n x 3")
+    H = _lib.hip()
+    fp = POINTER(c_float)
+    params = np.zeros(N_PARAMS, dtype=np.float32)
+    norm = np.zeros(16, dtype=np.float32)
+    every = int(log_every) if log_every else 0
+    trace = np.zeros(max(1, (steps + every - 1) // every) if every else 1, dtype=np.float32)
+    full = c_float(0)
+    rc = H.icempm_mlp_train(X.ctypes.data_as(fp), T.ctypes.data_as(fp), n, int(steps), int(batch), float(lr),
+                            int(seed) & 0xFFFFFFFFFFFFFFFF, params.ctypes.data_as(fp), norm.ctypes.data_as(fp),
+                            trace.ctypes.data_as(fp) if every else None, every, byref(full))
+    if rc != 0:
+        raise IcempmError(rc, "icempm_mlp_train failed")
+    losses = [(k * every, float(v)) for k, v in enumerate(trace)] if every else []
+    losses.append((steps, float(full.value)))
     return params, norm, losses

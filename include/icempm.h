@@ -242,7 +242,10 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
                           size_t pitch);
 int icempm_batch_finalize(icempm_batch *b);
 /* n substeps of every case from the batch's current time; use_graph != 0
- * captures the loop once per n and replays it */
+ * captures the loop once per n (and start, below) and replays it.  The last
+ * pass of a run leaves the next substep's P2G in the batch's grids, made
+ * from the points as the run leaves them, and the next run starts from it;
+ * after icempm_batch_restore or a re-sort the next run makes its own. */
 int icempm_batch_run(icempm_batch *b, int n, int use_graph);
 /* device time of the last icempm_batch_run (HIP events on the batch stream;
  * a graph run is timed from its launch, after capture, instantiation and
@@ -357,6 +360,19 @@ const float *icempm_mlp_device_params(const icempm_mlp *m);
 int icempm_mlp_step(icempm_mlp *m, const float *X, const float *T, int r0, int B, float lr,
                     float *grad_out, float *loss_out);
 int icempm_mlp_infer(icempm_mlp *m, const float *X, int n, float *Y);
+/* The whole training of the learned map from harvested pairs (host arrays:
+ * X n x 5 features, T n x 3 targets, as icempm_batch_read_harvest returns
+ * them): fits the normalisation (norm_out: 16 floats, the layout
+ * icempm_batch_set_return_map takes), shuffles the rows once (seeded),
+ * draws Glorot-scaled initial weights (seeded), runs `steps` Adam steps on
+ * consecutive windows of `batch` rows with the learning rate decaying from
+ * lr to lr / 20 along a cosine, and writes the parameters to params_out
+ * (icempm_mlp_num_params() floats).  trace (may be NULL) receives the window
+ * loss of every trace_every-th step; final_loss (may be NULL) the loss over
+ * all rows. */
+int icempm_mlp_train(const float *X, const float *T, int n, int steps, int batch, float lr,
+                     unsigned long long seed, float *params_out, float *norm_out, float *trace,
+                     int trace_every, float *final_loss);
 /* plain device memory for training data and tests */
 int icempm_device_alloc(size_t nbytes, void **ptr);
 void icempm_device_free(void *ptr);

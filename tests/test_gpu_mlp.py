@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from icempm_amd.learned import MLP, DeviceArray, init_params, mfma_probe
+from icempm_amd.learned import MLP, DeviceArray, init_params, mfma_probe, train_return_map
 
 pytestmark = pytest.mark.gpu
 
@@ -108,3 +108,30 @@ def test_inference_matches_bf16_emulation(gpu_available):
     err = np.abs(Y - ref)
     assert np.quantile(err, 0.99) <= 1e-4 * max(1.0, np.abs(ref).max())
     assert err.max() <= 2e-2 * max(1.0, np.abs(ref).max())
+
+
+def test_training_driver(gpu_available):
+    """icempm_mlp_train (the library's training driver, which
+    learned.train_return_map calls): the normalisation it fits is the
+    population mean and standard deviation of every column (a constant input
+    column gets mean 0 and scale 1), training lowers the loss, and a seed
+    fixes the result bit for bit (the gradient partials are summed in a fixed
+    order)."""
+    print("mlp training driver", flush=True)
+    rng = np.random.default_rng(7)
+    X, T = data(rng, 20000)
+    X[:, 3] = 0.25                               # a constant feature
+    p1, norm, losses = train_return_map(X, T, steps=400, batch=2048, lr=3e-3, seed=11, log_every=100)
+    xm, xs = X.astype(np.float64).mean(0), X.astype(np.float64).std(0)
+    xm[3], xs[3] = 0.0, 1.0
+    tm, ts = T.astype(np.float64).mean(0), T.astype(np.float64).std(0)
+    ref = np.concatenate([xm, xs, tm, ts])
+    assert np.allclose(norm, ref, rtol=1e-5, atol=1e-6), (norm, ref)
+    print("losses", losses, flush=True)
+    assert [s for s, _ in losses] == [0, 100, 200, 300, 400]
+    assert losses[-1][1] < 0.5 * losses[0][1]
+    p2, norm2, _ = train_return_map(X, T, steps=400, batch=2048, lr=3e-3, seed=11)
+    assert np.array_equal(p1.view(np.uint32), p2.view(np.uint32))
+    assert np.array_equal(norm, norm2)
+    p3, _, _ = train_return_map(X, T, steps=400, batch=2048, lr=3e-3, seed=12)
+    assert not np.array_equal(p1, p3)
