@@ -132,7 +132,7 @@ HIP_EXPORTS = [
     "icempm_write_grid", "icempm_read_bins", "icempm_set_tuning", "icempm_get_tuning",
     "icempm_measure_copy", "icempm_batch_create", "icempm_batch_destroy",
     "icempm_batch_last_error", "icempm_batch_add_case", "icempm_batch_finalize",
-    "icempm_batch_run", "icempm_batch_last_ms", "icempm_batch_num_points",
+    "icempm_batch_run", "icempm_batch_prepare", "icempm_batch_last_ms", "icempm_batch_num_points",
     "icempm_batch_force_history", "icempm_batch_force_bins", "icempm_batch_read_points", "icempm_batch_error_flag",
     "icempm_batch_set_return_map", "icempm_batch_read_harvest", "icempm_batch_map_stats",
     "icempm_batch_map_branch_stats", "icempm_batch_set_map_branches", "icempm_batch_get_map_branches",
@@ -220,6 +220,7 @@ def hip():
     _sig(L, "icempm_batch_add_case", c_int, P, c_void_p, c_size_t, POINTER(c_double), c_size_t)
     _sig(L, "icempm_batch_finalize", c_int, P)
     _sig(L, "icempm_batch_run", c_int, P, c_int, c_int)
+    _sig(L, "icempm_batch_prepare", c_int, P, c_int)
     _sig(L, "icempm_batch_last_ms", c_int, P, POINTER(c_float))
     _sig(L, "icempm_batch_num_points", c_int, P)
     _sig(L, "icempm_batch_force_history", c_int, P, c_int, POINTER(c_double), c_int)

@@ -91,6 +91,11 @@ class Batch:
     def finalize(self) -> None:
         self._check(self._H.icempm_batch_finalize(self._b))
 
+    def prepare(self, n: int) -> None:
+        """Capture, instantiate and upload the graphs of an n-substep run ahead
+        of time (icempm_batch_prepare)."""
+        self._check(self._H.icempm_batch_prepare(self._b, n))
+
     def run(self, n: int, use_graph: bool = True) -> float:
         """n substeps of every case; returns the device time in ms."""
         self._check(self._H.icempm_batch_run(self._b, n, 1 if use_graph else 0))

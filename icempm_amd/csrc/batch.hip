@@ -1748,37 +1748,60 @@ namespace {
 // P2G the previous run left (b->pending) or from a fresh P2G, as one graph
 // launch (captured when the start, the step count or the pointers change)
 // or as stream launches; returns the device time in ms.
+// The executable graph of n substeps from the given start (-1 fresh, 0 or 1:
+// the P2G in grid A or B) with these time-table and history rows, captured,
+// instantiated and put on the device unless its slot already holds it.
+int ensure_graph(icempm_batch *b, int n, const double *tTab, double *hist, int start) {
+    icempm_batch::GraphSlot &g = b->graphs[start + 1];
+    if (g.exec && g.steps == n && g.T == tTab && g.H == hist) return ICEMPM_OK;
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = icempm_batch::GraphSlot{};
+    hipGraph_t graph = nullptr;
+    int end = -1;
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch capture");
+    hipError_t e = enqueue_loop(b, n, tTab, hist, start, &end);
+    hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return bfail(b, ICEMPM_E_HIP, std::string("batch capture: ") + hipGetErrorString(e));
+    }
+    BCHK(b, e2, "batch capture");
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    BCHK(b, e, "batch graph instantiate");
+    g.steps = n;
+    g.T = tTab;
+    g.H = hist;
+    g.end = end;
+    // put the executable graph on the device now, so that its first launch
+    // costs what a replay costs
+    BCHK(b, hipGraphUpload(g.exec, b->stream), "batch graph upload");
+    BCHK(b, hipStreamSynchronize(b->stream), "batch graph upload");
+    return ICEMPM_OK;
+}
+
+// the time-table and history buffers for runs of up to n substeps
+int ensure_capacity(icempm_batch *b, int n) {
+    if (n <= b->cap) return ICEMPM_OK;
+    drop_graph(b);
+    dfree(b->tTab);
+    dfree(b->hist);
+    b->cap = 0;
+    BCHK(b, hipMalloc(&b->tTab, (size_t)n * sizeof(double)), "batch run");
+    BCHK(b, hipMalloc(&b->hist, 2 * b->hc.size() * (size_t)n * sizeof(double)), "batch run");
+    b->cap = n;
+    return ICEMPM_OK;
+}
+
 int run_segment(icempm_batch *b, int n, int use_graph, const double *tTab, double *hist, float *ms) {
     b->tevNext = 0;
     const int start = b->pending;
     b->pending = -1;   // until the loop below is enqueued whole
     int end = -1;
     if (use_graph) {
-        icempm_batch::GraphSlot &g = b->graphs[start + 1];
-        if (!g.exec || g.steps != n || g.T != tTab || g.H != hist) {
-            if (g.exec) (void)hipGraphExecDestroy(g.exec);
-            g = icempm_batch::GraphSlot{};
-            hipGraph_t graph = nullptr;
-            BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch capture");
-            hipError_t e = enqueue_loop(b, n, tTab, hist, start, &end);
-            hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
-            if (e != hipSuccess) {
-                if (graph) (void)hipGraphDestroy(graph);
-                return bfail(b, ICEMPM_E_HIP, std::string("batch capture: ") + hipGetErrorString(e));
-            }
-            BCHK(b, e2, "batch capture");
-            e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            BCHK(b, e, "batch graph instantiate");
-            g.steps = n;
-            g.T = tTab;
-            g.H = hist;
-            g.end = end;
-            // put the executable graph on the device now, so that its first
-            // launch (timed below) costs what a replay costs
-            BCHK(b, hipGraphUpload(g.exec, b->stream), "batch graph upload");
-            BCHK(b, hipStreamSynchronize(b->stream), "batch graph upload");
-        }
+        const int rc = ensure_graph(b, n, tTab, hist, start);
+        if (rc) return rc;
+        const icempm_batch::GraphSlot &g = b->graphs[start + 1];
         end = g.end;
         // timed from here: capture and instantiation are host work before it
         BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
@@ -1811,14 +1834,8 @@ int icempm_batch_run(icempm_batch *b, int n, int use_graph) {
     if (use_graph && b->timingStride > 0)
         return bfail(b, ICEMPM_E_STATE, "run: kernel timing needs stream launches (use_graph = 0)");
     const size_t nc = b->hc.size();
-    if (n > b->cap) {
-        drop_graph(b);
-        dfree(b->tTab);
-        dfree(b->hist);
-        BCHK(b, hipMalloc(&b->tTab, (size_t)n * sizeof(double)), "batch run");
-        BCHK(b, hipMalloc(&b->hist, 2 * nc * (size_t)n * sizeof(double)), "batch run");
-        b->cap = n;
-    }
+    int rc0 = ensure_capacity(b, n);
+    if (rc0) return rc0;
     // the time of each substep by repeated addition (model.cpp:23)
     std::vector<double> tt((size_t)n);
     double t = b->t;
@@ -1853,6 +1870,19 @@ int icempm_batch_run(icempm_batch *b, int n, int use_graph) {
     b->t = t;
     b->step += n;
     b->lastRun = n;
+    return ICEMPM_OK;
+}
+
+int icempm_batch_prepare(icempm_batch *b, int n) {
+    if (!b || n <= 0) return ICEMPM_E_ARG;
+    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "prepare: batch not finalized");
+    BCHK(b, hipStreamSynchronize(b->stream), "prepare");
+    int rc = ensure_capacity(b, n);
+    if (rc) return rc;
+    // the first segment of such a run, from each of the three starts
+    const int len = b->resortEvery > 0 ? std::min(b->resortEvery, n) : n;
+    for (int start = -1; start <= 1; ++start)
+        if ((rc = ensure_graph(b, len, b->tTab, b->hist, start))) return rc;
     return ICEMPM_OK;
 }
 

@@ -247,6 +247,13 @@ int icempm_batch_finalize(icempm_batch *b);
  * from the points as the run leaves them, and the next run starts from it;
  * after icempm_batch_restore or a re-sort the next run makes its own. */
 int icempm_batch_run(icempm_batch *b, int n, int use_graph);
+/* Host set-up of graph runs of n substeps ahead of time: the time-table and
+ * history buffers, and the graphs of such a run from each of its possible
+ * starts (fresh, or continuing a previous run), captured, instantiated and
+ * uploaded.  A run of n substeps then launches at once, whatever runs come
+ * between (shorter runs keep the buffers; a run of another length from the
+ * same start replaces that start's graph). */
+int icempm_batch_prepare(icempm_batch *b, int n);
 /* device time of the last icempm_batch_run (HIP events on the batch stream;
  * a graph run is timed from its launch, after capture, instantiation and
  * hipGraphUpload of the graph) */

@@ -118,6 +118,7 @@ def test_runs_continue_from_the_previous_runs_p2g(gpu_available, use_graph):
 
     go("continued")
     b.restore()
+    b.prepare(13)           # graphs of the 13-substep run captured ahead: the same results
     go("after a restore")
     b.restore()
     go("re-sorted between runs", between=b.resort)
