@@ -54,10 +54,11 @@ BATCH_F64_BYTES_PER_POINT = 212
 # writes them; Jp_inv is read only by crushed points, and it and the crushed
 # flag are written only when they change.  FP64 single case: read pos 16 +
 # Fe 32 + crushed 1 + grain 2, write pos 16 + Fe 32 = 99 B; batched FP32
-# storage (positions 48-bit fixed point in three 32-bit rows, no grain row):
-# read 12 + 16 + 1, write 12 + 16 = 57 B.
+# storage (a 16-byte record of the 48-bit fixed-point position and the
+# crushed flag, a 16-byte record of Fe - I, no grain row): read 16 + 16,
+# write 16 + 16 = 64 B.
 FUSED_MOVED_BYTES_PER_POINT = 99
-BATCH_F32_MOVED_BYTES_PER_POINT = 57
+BATCH_F32_MOVED_BYTES_PER_POINT = 64
 # With FP32 point storage the batched gathers read each node's velocity as
 # an FP32 pair: 24 B written by the P2G + 8 B read per node.
 BATCH_F32_MOVED_BYTES_PER_NODE = 32

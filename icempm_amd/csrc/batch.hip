@@ -66,8 +66,11 @@ struct Rows {
     // grid velocity as the gathers read it: FP64 rows 1-2 of the grid, or
     // (FP32 storage) one FP32 pair per node written by the grid update
     using G = typename std::conditional<F32, float2, double>::type;
-    static constexpr int kPosRows = F32 ? 3 : 2;
-    P *pos;            // kPosRows rows
+    // FP64: rows x, y.  FP32: one 16-byte record per slot, {high 32 bits of
+    // x, of y, the two low 16-bit halves, the crushed flag} (one load and one
+    // store per pass; the crushed row is then unused)
+    static constexpr int kPosRows = F32 ? 4 : 2;
+    P *pos;            // kPosRows values per slot
     V *rest;           // 11 rows: vel(2) Fe(4) Bp(4) Jp(1)
     unsigned char *crushed;
     short *grain;      // null when every case is uniform
@@ -121,22 +124,20 @@ static __device__ __forceinline__ void pos_load(const double *pos, size_t pp, in
         y = *at(pos + pp, u);
     }
 }
+typedef unsigned int prec4 __attribute__((ext_vector_type(4)));
+// FP32 storage: the slot's record; returns its crushed flag
 template <bool NT>
-static __device__ __forceinline__ void pos_load(const unsigned int *pos, size_t pp, int pt, const CaseParams &cp,
-                                                double &x, double &y) {
-    const unsigned u = (unsigned)pt;
-    unsigned int hx, hy, lo;
-    if constexpr (NT) {
-        hx = __builtin_nontemporal_load(at(pos, u));
-        hy = __builtin_nontemporal_load(at(pos + pp, u));
-        lo = __builtin_nontemporal_load(at(pos + 2 * pp, u));
-    } else {
-        hx = *at(pos, u);
-        hy = *at(pos + pp, u);
-        lo = *at(pos + 2 * pp, u);
-    }
-    x = fix48_get(hx, lo & 0xFFFFu, cp);
-    y = fix48_get(hy, lo >> 16, cp);
+static __device__ __forceinline__ unsigned pos_load(const unsigned int *pos, size_t, int pt, const CaseParams &cp,
+                                                    double &x, double &y) {
+    const prec4 *rec = reinterpret_cast<const prec4 *>(pos) + pt;
+    prec4 v;
+    if constexpr (NT)
+        v = __builtin_nontemporal_load(rec);
+    else
+        v = *rec;
+    x = fix48_get(v.x, v.z & 0xFFFFu, cp);
+    y = fix48_get(v.y, v.z >> 16, cp);
+    return v.w & 0xFFu;
 }
 // stores (x, y) and replaces them by the values the rows now hold.  FP64
 // rows are stored through the L2 (a wavefront's 512 B of a row straddle
@@ -144,20 +145,19 @@ static __device__ __forceinline__ void pos_load(const unsigned int *pos, size_t 
 // than non-temporal stores); the FP32 rows' 256 B stream non-temporally
 // (measured faster that way)
 static __device__ __forceinline__ void pos_store(double *pos, size_t pp, int pt, const CaseParams &, double &x,
-                                                 double &y) {
+                                                 double &y, unsigned) {
     const unsigned u = (unsigned)pt;
     *at(pos, u) = x;
     *at(pos + pp, u) = y;
 }
-static __device__ __forceinline__ void pos_store(unsigned int *pos, size_t pp, int pt, const CaseParams &cp,
-                                                 double &x, double &y) {
-    const unsigned u = (unsigned)pt;
+// FP32 storage: the whole record, with the crushed flag
+static __device__ __forceinline__ void pos_store(unsigned int *pos, size_t, int pt, const CaseParams &cp,
+                                                 double &x, double &y, unsigned crushed) {
     unsigned int hx, lx, hy, ly;
     fix48_put(x, cp, hx, lx);
     fix48_put(y, cp, hy, ly);
-    __builtin_nontemporal_store(hx, at(pos, u));
-    __builtin_nontemporal_store(hy, at(pos + pp, u));
-    __builtin_nontemporal_store(lx | (ly << 16), at(pos + 2 * pp, u));
+    const prec4 v = {hx, hy, lx | (ly << 16), crushed};
+    __builtin_nontemporal_store(v, reinterpret_cast<prec4 *>(pos) + pt);
     x = fix48_get(hx, lx, cp);
     y = fix48_get(hy, ly, cp);
 }
@@ -167,6 +167,15 @@ static __host__ __device__ inline double pos_round(double x, const CaseParams &c
     unsigned int h, l;
     fix48_put(x, cp, h, l);
     return fix48_get(h, l, cp);
+}
+
+// FP32 storage keeps a point's Fe - I as one 16-byte record in the space of
+// rest rows 2-5 (one float4 per point, one load and one store per pass
+// instead of four of each); FP64 storage keeps the four rows.
+typedef float fe4 __attribute__((ext_vector_type(4)));
+static __device__ __forceinline__ fe4 *fe_rec(float *R, size_t pp) { return reinterpret_cast<fe4 *>(R + pp * 2); }
+static __device__ __forceinline__ const fe4 *fe_rec(const float *R, size_t pp) {
+    return reinterpret_cast<const fe4 *>(R + pp * 2);
 }
 
 // the velocity of a node (index within its case's grid) as the gathers read it
@@ -355,12 +364,21 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         if (valid) {
             // streamed once per substep: non-temporal, so the grid keeps the L2
             const unsigned u = (unsigned)pt;
-            pos_load<true>(S.pos, pp, pt, cp, px, py);
-            F00 = fe0 + (double)__builtin_nontemporal_load(at(R + pp * 2, u));
-            F01 = (double)__builtin_nontemporal_load(at(R + pp * 3, u));
-            F10 = (double)__builtin_nontemporal_load(at(R + pp * 4, u));
-            F11 = fe0 + (double)__builtin_nontemporal_load(at(R + pp * 5, u));
-            crushed = __builtin_nontemporal_load(at(S.crushed, u));
+            if constexpr (F32) {
+                crushed = (unsigned char)pos_load<true>(S.pos, pp, pt, cp, px, py);
+                const fe4 f = __builtin_nontemporal_load(fe_rec(R, pp) + pt);
+                F00 = fe0 + (double)f.x;
+                F01 = (double)f.y;
+                F10 = (double)f.z;
+                F11 = fe0 + (double)f.w;
+            } else {
+                pos_load<true>(S.pos, pp, pt, cp, px, py);
+                F00 = fe0 + (double)__builtin_nontemporal_load(at(R + pp * 2, u));
+                F01 = (double)__builtin_nontemporal_load(at(R + pp * 3, u));
+                F10 = (double)__builtin_nontemporal_load(at(R + pp * 4, u));
+                F11 = fe0 + (double)__builtin_nontemporal_load(at(R + pp * 5, u));
+                crushed = __builtin_nontemporal_load(at(S.crushed, u));
+            }
             if (cp.grainVar != 0) grain = __builtin_nontemporal_load(at(grains, u));
         }
         int bx = 0, by = 0;
@@ -561,14 +579,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             }
             // px, py become what the rows hold: the P2G takes them at their
             // stored precision
-            pos_store(S.pos, pp, pt, cp, px, py);
+            pos_store(S.pos, pp, pt, cp, px, py, crushed);
             const unsigned u = (unsigned)pt;
             if (!defer) {
                 if constexpr (F32) {
-                    __builtin_nontemporal_store((V)(F00 - fe0), at(R + pp * 2, u));
-                    __builtin_nontemporal_store((V)F01, at(R + pp * 3, u));
-                    __builtin_nontemporal_store((V)F10, at(R + pp * 4, u));
-                    __builtin_nontemporal_store((V)(F11 - fe0), at(R + pp * 5, u));
+                    const fe4 f = {(float)(F00 - fe0), (float)F01, (float)F10, (float)(F11 - fe0)};
+                    __builtin_nontemporal_store(f, fe_rec(R, pp) + pt);
                 } else {   // as pos_store
                     *at(R + pp * 2, u) = (V)F00;
                     *at(R + pp * 3, u) = (V)F01;
@@ -590,7 +606,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 R[pp * 9 + pt] = (V)B11;
             }
             if (Jp != Jp0 && !defer) *at(R + pp * 10, u) = (V)(Jp - fe0);
-            if (crushed != crushed0) *at(S.crushed, u) = crushed;
+            if (!F32 && crushed != crushed0) *at(S.crushed, u) = crushed;   // FP32: in the record
             if constexpr (F32) {
                 // Fe is what the next substep reads back: the P2G takes it at
                 // its stored precision.  v and Bp stay FP64: the next G2P
@@ -609,10 +625,18 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 pos_load<false>(S.pos, pp, pt, cp, px, py);
                 vx = (double)R[pt];
                 vy = (double)R[pp + pt];
-                F00 = fe0 + (double)R[pp * 2 + pt];
-                F01 = (double)R[pp * 3 + pt];
-                F10 = (double)R[pp * 4 + pt];
-                F11 = fe0 + (double)R[pp * 5 + pt];
+                if constexpr (F32) {
+                    const fe4 f = fe_rec(R, pp)[pt];
+                    F00 = fe0 + (double)f.x;
+                    F01 = (double)f.y;
+                    F10 = (double)f.z;
+                    F11 = fe0 + (double)f.w;
+                } else {
+                    F00 = fe0 + (double)R[pp * 2 + pt];
+                    F01 = (double)R[pp * 3 + pt];
+                    F10 = (double)R[pp * 4 + pt];
+                    F11 = fe0 + (double)R[pp * 5 + pt];
+                }
                 B00 = (double)R[pp * 6 + pt];
                 B01 = (double)R[pp * 7 + pt];
                 B10 = (double)R[pp * 8 + pt];
@@ -888,10 +912,14 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
         }
         if (have) {
             const int pt = e.pt;
-            R[pp * 2 + pt] = (V)(F00 - fe0);
-            R[pp * 3 + pt] = (V)F01;
-            R[pp * 4 + pt] = (V)F10;
-            R[pp * 5 + pt] = (V)(F11 - fe0);
+            if constexpr (F32) {
+                fe_rec(R, pp)[pt] = fe4{(float)(F00 - fe0), (float)F01, (float)F10, (float)(F11 - fe0)};
+            } else {
+                R[pp * 2 + pt] = (V)(F00 - fe0);
+                R[pp * 3 + pt] = (V)F01;
+                R[pp * 4 + pt] = (V)F10;
+                R[pp * 5 + pt] = (V)(F11 - fe0);
+            }
             if (Jp != e.Jp) R[pp * 10 + pt] = (V)(Jp - fe0);
             if (gridOut) scatter_point<F32>(e, cp, k, F00, F01, F10, F11, gridOut, gpitch);
         }
@@ -1048,8 +1076,17 @@ __global__ void k_batch_permute(Rows<F32> src, Rows<F32> dst, const int *__restr
     if (k >= nPts) return;
     const int j = perm[k];
     const size_t p = src.pitch;
-    for (int r = 0; r < Rows<F32>::kPosRows; ++r) dst.pos[r * p + k] = src.pos[r * p + j];
-    for (int r = 0; r < 11; ++r) dst.rest[r * p + k] = src.rest[r * p + j];
+    if constexpr (F32)
+        reinterpret_cast<prec4 *>(dst.pos)[k] = reinterpret_cast<const prec4 *>(src.pos)[j];
+    else
+        for (int r = 0; r < Rows<F32>::kPosRows; ++r) dst.pos[r * p + k] = src.pos[r * p + j];
+    if constexpr (F32) {
+        for (int r = 0; r < 11; ++r)
+            if (r < 2 || r > 5) dst.rest[r * p + k] = src.rest[r * p + j];
+        fe_rec(dst.rest, p)[k] = fe_rec(static_cast<const float *>(src.rest), p)[j];
+    } else {
+        for (int r = 0; r < 11; ++r) dst.rest[r * p + k] = src.rest[r * p + j];
+    }
     dst.crushed[k] = src.crushed[j];
     if (src.grain) dst.grain[k] = src.grain[j];
     dstOrig[k] = srcOrig[j];
@@ -1367,6 +1404,14 @@ size_t pos_bytes(const icempm_batch *b) {
 
 size_t rest_bytes(const icempm_batch *b) {
     return 11 * b->pitch * (b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double));
+}
+
+// Element of rest value r (v0, v1, Fe00, Fe01, Fe10, Fe11, Bp00..Bp11, Jp_inv)
+// of a slot: a row each, except FP32 storage's Fe, one 4-float record per
+// slot in the space of rows 2-5.
+size_t rest_elem(const icempm_batch *b, int r, size_t slot) {
+    if (b->precision == ICEMPM_BATCH_F32 && r >= 2 && r <= 5) return 2 * b->pitch + 4 * slot + (size_t)(r - 2);
+    return (size_t)r * b->pitch + slot;
 }
 
 // Stable re-sort of every case's points by their current cell, then a new
@@ -1691,17 +1736,18 @@ int icempm_batch_finalize(icempm_batch *b) {
                 if (b->precision == ICEMPM_BATCH_F32) {
                     if (r == 2 || r == 5 || r == 10) v -= 1.0;    // Fe00, Fe11, Jp_inv stored less 1
                     float f = (float)v;
-                    std::memcpy(&hrest[((size_t)r * b->pitch + o + k) * vs], &f, vs);
+                    std::memcpy(&hrest[rest_elem(b, r, (size_t)(o + k)) * vs], &f, vs);
                 } else {
-                    std::memcpy(&hrest[((size_t)r * b->pitch + o + k) * vs], &v, vs);
+                    std::memcpy(&hrest[rest_elem(b, r, (size_t)(o + k)) * vs], &v, vs);
                 }
             }
         }
         hcp.push_back(h.cp);
     }
     if (b->precision == ICEMPM_BATCH_F32) {
-        // 48-bit fixed point: high words of x and y, then both low halves
-        std::vector<unsigned int> q(3 * b->pitch, 0u);
+        // one record per slot: 48-bit fixed point (high words of x and y,
+        // both low halves) and the crushed flag
+        std::vector<unsigned int> q(4 * b->pitch, 0u);
         for (size_t ci = 0; ci < hcp.size(); ++ci) {
             const CaseParams &cp = hcp[ci];
             for (int k = 0; k < cp.nPts; ++k) {
@@ -1709,9 +1755,10 @@ int icempm_batch_finalize(icempm_batch *b) {
                 unsigned int hx, lx, hy, ly;
                 fix48_put(hpos[i], cp, hx, lx);
                 fix48_put(hpos[b->pitch + i], cp, hy, ly);
-                q[i] = hx;
-                q[b->pitch + i] = hy;
-                q[2 * b->pitch + i] = lx | (ly << 16);
+                q[4 * i] = hx;
+                q[4 * i + 1] = hy;
+                q[4 * i + 2] = lx | (ly << 16);
+                q[4 * i + 3] = hcr[i];
             }
         }
         BCHK(b, hipMemcpy(b->pos, q.data(), q.size() * sizeof(unsigned int), hipMemcpyHostToDevice), "batch upload");
@@ -1944,15 +1991,15 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
     std::vector<double> hpos(2 * (size_t)n);
     std::vector<unsigned char> hrest(11 * (size_t)n * vs), hcr(n);
     if (b->precision == ICEMPM_BATCH_F32) {
-        std::vector<unsigned int> q(3 * (size_t)n);
+        std::vector<unsigned int> q(4 * (size_t)n);
         const unsigned int *P = static_cast<const unsigned int *>(b->pos);
-        for (int r = 0; r < 3; ++r)
-            BCHK(b, hipMemcpy(q.data() + (size_t)r * n, P + r * b->pitch + o, n * sizeof(unsigned int),
-                              hipMemcpyDeviceToHost), "read_points");
+        BCHK(b, hipMemcpy(q.data(), P + 4 * (size_t)o, 4 * (size_t)n * sizeof(unsigned int), hipMemcpyDeviceToHost),
+             "read_points");
         for (int i = 0; i < n; ++i) {
-            const unsigned int lo = q[2 * (size_t)n + i];
-            hpos[i] = fix48_get(q[i], lo & 0xFFFFu, cp);
-            hpos[n + i] = fix48_get(q[(size_t)n + i], lo >> 16, cp);
+            const unsigned int lo = q[4 * (size_t)i + 2];
+            hpos[i] = fix48_get(q[4 * (size_t)i], lo & 0xFFFFu, cp);
+            hpos[n + i] = fix48_get(q[4 * (size_t)i + 1], lo >> 16, cp);
+            hcr[i] = (unsigned char)(q[4 * (size_t)i + 3] & 0xFFu);
         }
     } else {
         const double *P = static_cast<const double *>(b->pos);
@@ -1960,10 +2007,23 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
         BCHK(b, hipMemcpy(hpos.data() + n, P + b->pitch + o, n * sizeof(double), hipMemcpyDeviceToHost),
              "read_points");
     }
-    for (int r = 0; r < 11; ++r)
-        BCHK(b, hipMemcpy(hrest.data() + (size_t)r * n * vs, (unsigned char *)b->rest + ((size_t)r * b->pitch + o) * vs,
+    // rows of the case's slots [o, o + n) into hrest[r][slot]; FP32 storage's
+    // Fe records come down as one block and are split
+    for (int r = 0; r < 11; ++r) {
+        if (b->precision == ICEMPM_BATCH_F32 && r >= 2 && r <= 5) continue;
+        BCHK(b, hipMemcpy(hrest.data() + (size_t)r * n * vs, (unsigned char *)b->rest + rest_elem(b, r, o) * vs,
                           n * vs, hipMemcpyDeviceToHost), "read_points");
-    BCHK(b, hipMemcpy(hcr.data(), b->crushed + o, n, hipMemcpyDeviceToHost), "read_points");
+    }
+    if (b->precision == ICEMPM_BATCH_F32) {
+        std::vector<float> fe(4 * (size_t)n);
+        BCHK(b, hipMemcpy(fe.data(), (unsigned char *)b->rest + rest_elem(b, 2, o) * vs, 4 * (size_t)n * vs,
+                          hipMemcpyDeviceToHost), "read_points");
+        for (int slot = 0; slot < n; ++slot)
+            for (int r = 2; r <= 5; ++r)
+                std::memcpy(&hrest[((size_t)r * n + slot) * vs], &fe[4 * (size_t)slot + (r - 2)], vs);
+    }
+    if (b->precision != ICEMPM_BATCH_F32)   // FP32: from the position records above
+        BCHK(b, hipMemcpy(hcr.data(), b->crushed + o, n, hipMemcpyDeviceToHost), "read_points");
     std::vector<short> hgr((size_t)n, 0);
     if (b->grain) BCHK(b, hipMemcpy(hgr.data(), b->grain + o, n * sizeof(short), hipMemcpyDeviceToHost), "read_points");
     std::vector<int> horig((size_t)n);

@@ -227,7 +227,8 @@ int icempm_read_bins(icempm_ctx *ctx, double *out);
  * Precision: ICEMPM_BATCH_F64 stores the point state in FP64;
  * ICEMPM_BATCH_F32 stores velocity, Fe - I, Bp and Jp_inv - 1 in FP32 and the
  * positions as 48-bit fixed point over the case's grid width (a quantum of
- * GridX * cellsize / 2^48); the grid sums stay FP64, the grid velocities
+ * GridX * cellsize / 2^48), each point's position (with its crushed flag) and
+ * its Fe - I as one 16-byte record apiece; the grid sums stay FP64, the grid velocities
  * reach the gathers as FP32 pairs, and all arithmetic is FP64.  The
  * per-substep total indenter force of every case is recorded, and its
  * angular bins (Indenter_2D, gpu_implementation5.cu:262-269) summed over
