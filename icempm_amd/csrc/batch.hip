@@ -67,8 +67,9 @@ struct Rows {
     // (FP32 storage) one FP32 pair per node written by the grid update
     using G = typename std::conditional<F32, float2, double>::type;
     // FP64: rows x, y.  FP32: one 16-byte record per slot, {high 32 bits of
-    // x, of y, the two low 16-bit halves, the crushed flag} (one load and one
-    // store per pass; the crushed row is then unused)
+    // x, of y, the two low 16-bit halves, the crushed flag | grain id << 16}
+    // (one load and one store per pass; the crushed and grain rows are then
+    // unused)
     static constexpr int kPosRows = F32 ? 4 : 2;
     P *pos;            // kPosRows values per slot
     V *rest;           // 11 rows: vel(2) Fe(4) Bp(4) Jp(1)
@@ -125,7 +126,8 @@ static __device__ __forceinline__ void pos_load(const double *pos, size_t pp, in
     }
 }
 typedef unsigned int prec4 __attribute__((ext_vector_type(4)));
-// FP32 storage: the slot's record; returns its crushed flag
+// FP32 storage: the slot's record; returns its last word (crushed flag in
+// bits 0-7, grain id in bits 16-31)
 template <bool NT>
 static __device__ __forceinline__ unsigned pos_load(const unsigned int *pos, size_t, int pt, const CaseParams &cp,
                                                     double &x, double &y) {
@@ -137,7 +139,7 @@ static __device__ __forceinline__ unsigned pos_load(const unsigned int *pos, siz
         v = *rec;
     x = fix48_get(v.x, v.z & 0xFFFFu, cp);
     y = fix48_get(v.y, v.z >> 16, cp);
-    return v.w & 0xFFu;
+    return v.w;
 }
 // stores (x, y) and replaces them by the values the rows now hold.  FP64
 // rows are stored through the L2 (a wavefront's 512 B of a row straddle
@@ -150,13 +152,13 @@ static __device__ __forceinline__ void pos_store(double *pos, size_t pp, int pt,
     *at(pos, u) = x;
     *at(pos + pp, u) = y;
 }
-// FP32 storage: the whole record, with the crushed flag
+// FP32 storage: the whole record, its last word w as pos_load returns it
 static __device__ __forceinline__ void pos_store(unsigned int *pos, size_t, int pt, const CaseParams &cp,
-                                                 double &x, double &y, unsigned crushed) {
+                                                 double &x, double &y, unsigned w) {
     unsigned int hx, lx, hy, ly;
     fix48_put(x, cp, hx, lx);
     fix48_put(y, cp, hy, ly);
-    const prec4 v = {hx, hy, lx | (ly << 16), crushed};
+    const prec4 v = {hx, hy, lx | (ly << 16), w};
     __builtin_nontemporal_store(v, reinterpret_cast<prec4 *>(pos) + pt);
     x = fix48_get(hx, lx, cp);
     y = fix48_get(hy, ly, cp);
@@ -365,7 +367,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             // streamed once per substep: non-temporal, so the grid keeps the L2
             const unsigned u = (unsigned)pt;
             if constexpr (F32) {
-                crushed = (unsigned char)pos_load<true>(S.pos, pp, pt, cp, px, py);
+                const unsigned w = pos_load<true>(S.pos, pp, pt, cp, px, py);
+                crushed = (unsigned char)(w & 0xFFu);
+                grain = (short)(w >> 16);
                 const fe4 f = __builtin_nontemporal_load(fe_rec(R, pp) + pt);
                 F00 = fe0 + (double)f.x;
                 F01 = (double)f.y;
@@ -378,8 +382,8 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 F10 = (double)__builtin_nontemporal_load(at(R + pp * 4, u));
                 F11 = fe0 + (double)__builtin_nontemporal_load(at(R + pp * 5, u));
                 crushed = __builtin_nontemporal_load(at(S.crushed, u));
+                if (cp.grainVar != 0) grain = __builtin_nontemporal_load(at(grains, u));
             }
-            if (cp.grainVar != 0) grain = __builtin_nontemporal_load(at(grains, u));
         }
         int bx = 0, by = 0;
         double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
@@ -579,7 +583,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             }
             // px, py become what the rows hold: the P2G takes them at their
             // stored precision
-            pos_store(S.pos, pp, pt, cp, px, py, crushed);
+            pos_store(S.pos, pp, pt, cp, px, py, (unsigned)crushed | ((unsigned)(unsigned short)grain << 16));
             const unsigned u = (unsigned)pt;
             if (!defer) {
                 if constexpr (F32) {
@@ -1758,7 +1762,7 @@ int icempm_batch_finalize(icempm_batch *b) {
                 q[4 * i] = hx;
                 q[4 * i + 1] = hy;
                 q[4 * i + 2] = lx | (ly << 16);
-                q[4 * i + 3] = hcr[i];
+                q[4 * i + 3] = hcr[i] | ((unsigned int)(unsigned short)hgr[i] << 16);
             }
         }
         BCHK(b, hipMemcpy(b->pos, q.data(), q.size() * sizeof(unsigned int), hipMemcpyHostToDevice), "batch upload");
@@ -1990,6 +1994,7 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
     const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
     std::vector<double> hpos(2 * (size_t)n);
     std::vector<unsigned char> hrest(11 * (size_t)n * vs), hcr(n);
+    std::vector<short> hgr((size_t)n, 0);
     if (b->precision == ICEMPM_BATCH_F32) {
         std::vector<unsigned int> q(4 * (size_t)n);
         const unsigned int *P = static_cast<const unsigned int *>(b->pos);
@@ -2000,6 +2005,7 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
             hpos[i] = fix48_get(q[4 * (size_t)i], lo & 0xFFFFu, cp);
             hpos[n + i] = fix48_get(q[4 * (size_t)i + 1], lo >> 16, cp);
             hcr[i] = (unsigned char)(q[4 * (size_t)i + 3] & 0xFFu);
+            hgr[i] = (short)(q[4 * (size_t)i + 3] >> 16);
         }
     } else {
         const double *P = static_cast<const double *>(b->pos);
@@ -2024,8 +2030,8 @@ int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) 
     }
     if (b->precision != ICEMPM_BATCH_F32)   // FP32: from the position records above
         BCHK(b, hipMemcpy(hcr.data(), b->crushed + o, n, hipMemcpyDeviceToHost), "read_points");
-    std::vector<short> hgr((size_t)n, 0);
-    if (b->grain) BCHK(b, hipMemcpy(hgr.data(), b->grain + o, n * sizeof(short), hipMemcpyDeviceToHost), "read_points");
+    if (b->grain && b->precision != ICEMPM_BATCH_F32)
+        BCHK(b, hipMemcpy(hgr.data(), b->grain + o, n * sizeof(short), hipMemcpyDeviceToHost), "read_points");
     std::vector<int> horig((size_t)n);
     BCHK(b, hipMemcpy(horig.data(), b->orig + o, n * sizeof(int), hipMemcpyDeviceToHost), "read_points");
     std::memset(soa, 0, 16 * pitch * sizeof(double));
