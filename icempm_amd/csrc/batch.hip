@@ -400,12 +400,23 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             GT n0[9], n1[9];
             if (bx + 2 < gx && by + 2 < gy) {
                 if constexpr (F32) {
-                    // one FP32 pair per node, a row of three at a time
+                    // a stencil row's three FP32 pairs as one 16-byte load
+                    // (8-byte aligned: the unaligned 16-byte access global
+                    // memory allows) and one 8-byte load: 6 loads instead of
+                    // 9 (measured 2% off the pass)
+                    typedef float v4u __attribute__((ext_vector_type(4), aligned(8)));
 #pragma unroll
-                    for (int j = 0; j < 3; ++j)
-#pragma unroll
-                        for (int i = 0; i < 3; ++i)
-                            grid_v(gV, bx + i + (by + j) * gx, gpitch, n0[i * 3 + j], n1[i * 3 + j]);
+                    for (int j = 0; j < 3; ++j) {
+                        const unsigned node = (unsigned)(bx + (by + j) * gx);
+                        const v4u a = *reinterpret_cast<const v4u *>(at(gV, node));
+                        const float2 c = *at(gV, node + 2);
+                        n0[0 * 3 + j] = a.x;
+                        n1[0 * 3 + j] = a.y;
+                        n0[1 * 3 + j] = a.z;
+                        n1[1 * 3 + j] = a.w;
+                        n0[2 * 3 + j] = c.x;
+                        n1[2 * 3 + j] = c.y;
+                    }
                 } else {
                     const double *gvx = gV + gpitch + bx + by * gx;
                     const double *gvy = gvx + gpitch;
