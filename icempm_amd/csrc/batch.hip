@@ -776,15 +776,17 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             for (int ly = brow0; ly < bh; ly += bstep) {
                 const int lx = bcol;
                 const int e = ly * kWS + lx;
-                const double a = W0[e];
+                // the three sums in one LDS round trip; the atomics address
+                // the rows from their scalar bases with 32-bit offsets
+                const double a = W0[e], a1 = W1[e], a2 = W2[e];
                 if (a != 0) {
                     const int X = ox + lx, Y = oy + ly;
                     if (X < gx && Y < gy) {
-                        const int node = X + Y * gx;
+                        const unsigned node = (unsigned)(X + Y * gx);
                         const double m = cp.mass;   // the window holds sums per unit mass
-                        unsafeAtomicAdd(&gO[node], a * m);
-                        unsafeAtomicAdd(&gO[gpitch + node], W1[e] * m);
-                        unsafeAtomicAdd(&gO[2 * gpitch + node], W2[e] * m);
+                        unsafeAtomicAdd(at(gO, node), a * m);
+                        unsafeAtomicAdd(at(gO + gpitch, node), a1 * m);
+                        unsafeAtomicAdd(at(gO + 2 * gpitch, node), a2 * m);
                     }
                 }
             }
