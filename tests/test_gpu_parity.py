@@ -2,7 +2,7 @@
 
 ORDERED mode must reproduce the oracle bit for bit (same operation order,
 correctly rounded FP64).  FAST mode differs only in the order of the P2G and
-indenter-force sums; it is held to the north-star tolerance (indenter force
+indenter-force sums; it is held to the FP64 tolerance (indenter force
 history relative L2 <= 1e-6 in FP64) over the pre-fracture horizon and to
 tight per-field bounds.
 """
@@ -132,7 +132,7 @@ def test_fast_within_tolerance(gpu_available, case, nsub):
     assert err == 0 and m.error_flag() == 0
     n = case.n_points
     dev = m.points()
-    # north-star tolerance for FP64: indenter force history within 1e-6 rel L2
+    # the FP64 tolerance: indenter force history within 1e-6 rel L2
     assert rel_l2(hist_dev, hist_ref) <= 1e-6
     assert rel_l2(m.bins(), bins) <= 1e-6
     # positions to ~1e-12 of the domain, deformation gradient to 1e-9
@@ -229,7 +229,7 @@ def test_call_order_errors(gpu_available):
 
 def test_full_size_case(gpu_available):
     """The 1.6M-point configuration itself: ORDERED bit-exact for a few
-    substeps, FAST within the north-star tolerance, and the size-independent
+    substeps, FAST within the FP64 tolerance, and the size-independent
     conservation of a P2G (total mass N m; total momentum m sum v, since the
     APIC affine term sums to zero over the stencil)."""
     case = INDENT_1M6
