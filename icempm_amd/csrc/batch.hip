@@ -22,11 +22,13 @@
 // A case's NACC strengths are uniform (constants per case) or vary per grain
 // id as GetParametersForGrain makes them (gpu_implementation5.cu:97-111).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -1085,6 +1087,31 @@ __global__ void k_batch_keys(const PT *__restrict__ pos, size_t pitch, int nPts,
     vals[i] = i;
 }
 
+// The case-local base cell of every slot, packed as column | row << 16
+// (finalize keeps every grid within 65536 columns and rows): what the chunk
+// table is built from, without the band key's divisions on the host.
+template <class PT>
+__global__ void k_batch_cells(const PT *__restrict__ pos, size_t pitch, int nPts,
+                              const CaseParams *__restrict__ cases, int nCases, unsigned *__restrict__ cells) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPts) return;
+    int lo = 0, hi = nCases - 1;   // the last case with ptOffset <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cases[mid].ptOffset <= i) lo = mid; else hi = mid - 1;
+    }
+    const CaseParams &cp = cases[lo];
+    double x, y;
+    pos_load<false>(pos, pitch, i, cp, x, y);
+    // the cell k_batch_keys keys the slot by (a NaN position: the last cell)
+    unsigned cx = (unsigned)(cp.gx - 1), cy = (unsigned)(cp.gy - 1);
+    if (x == x && y == y) {
+        cx = (unsigned)fmin(fmax(x * cp.h_inv - 0.5, 0.0), (double)(cp.gx - 1));
+        cy = (unsigned)fmin(fmax(y * cp.h_inv - 0.5, 0.0), (double)(cp.gy - 1));
+    }
+    cells[i] = cx | cy << 16;
+}
+
 // Point state permuted: slot k of the destination <- slot perm[k] of the source.
 template <bool F32>
 __global__ void k_batch_permute(Rows<F32> src, Rows<F32> dst, const int *__restrict__ srcOrig,
@@ -1176,18 +1203,21 @@ struct icempm_batch {
     MapEntry *mapQ = nullptr;          // deferred points (learned map), nPts entries
     unsigned int *mapCount = nullptr;  // two counters, used alternately
 
-    // executable graphs of the substep loop, one per start: fresh (slot 0),
-    // or the first substep's P2G already in grid A (slot 1) or grid B (slot
-    // 2); each with the step count, time table and history rows it was
-    // captured with, and the grid that holds the next P2G after it
+    // executable graphs of the substep loop, two per start (of different
+    // step counts, the least recently used replaced): fresh (row 0), or the
+    // first substep's P2G already in grid A (row 1) or grid B (row 2); each
+    // with the step count, time table and history rows it was captured
+    // with, and the grid that holds the next P2G after it
     struct GraphSlot {
         hipGraphExec_t exec = nullptr;
         int steps = 0;
         const double *T = nullptr;
         double *H = nullptr;
         int end = -1;
+        unsigned long long used = 0;
     };
-    GraphSlot graphs[3];
+    GraphSlot graphs[3][2];
+    unsigned long long graphTick = 0;
     // the grid (0: A, 1: B) that holds the P2G of the next substep, made
     // from the current points by the last pass of the previous run; -1 when
     // there is none, or the points have changed since (restore, re-sort)
@@ -1203,6 +1233,9 @@ struct icempm_batch {
     int *orig = nullptr;                    // batch index of the point in each slot
     // re-sort scratch
     int *sKeys = nullptr, *sVals = nullptr, *sKeys2 = nullptr, *sVals2 = nullptr, *sTmp = nullptr;
+    // the slots' base cells for a new chunk table: on the device, and read
+    // back into pinned host memory
+    unsigned *dcells = nullptr, *hcells = nullptr;
     void *tpos = nullptr;
     void *trest = nullptr;
     unsigned char *tcrushed = nullptr;
@@ -1319,35 +1352,33 @@ hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const void *gV, doub
     return hipGetLastError();
 }
 
-// Chunk table from the case-local band keys of the base cells of the points
-// in slot order: chunks of <= chunkPoints consecutive points of one case
-// whose base cells span at most two cell rows and 126 columns, so that a
-// wavefront's window (4 node rows) holds all its stencils.  Points in band
-// order give chunks of one band each, until they move.
-void chunk_table(const icempm_batch *b, const std::vector<int> &keys, std::vector<int> &cstart,
+// Chunk table from the case-local base cells of the points in slot order
+// (column | row << 16): chunks of <= chunkPoints consecutive points of one
+// case whose base cells span at most two cell rows and 126 columns, so that
+// a wavefront's window (4 node rows) holds all its stencils, cut greedily.
+// Points in band order give chunks of one band each, until they move.  The
+// cases are independent, so they are cut on several host threads.
+void chunk_table(const icempm_batch *b, const unsigned *cells, std::vector<int> &cstart,
                  std::vector<int> &ccase) {
-    cstart.clear();
-    ccase.clear();
+    const size_t nc = b->hc.size();
     const int cap = b->chunkPoints;
-    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
+    std::vector<std::vector<int>> starts(nc);
+    auto cut = [&](size_t ci) {
         const CaseParams &cp = b->hc[ci].cp;
-        const int n = cp.nPts, gx = cp.gx, gy = cp.gy, o = cp.ptOffset;
-        if (n <= 0) continue;
+        const int n = cp.nPts, o = cp.ptOffset;
+        std::vector<int> &st = starts[ci];
+        if (n <= 0) return;
+        st.reserve((size_t)n / (size_t)cap + 16);
+        st.push_back(o);
         int cs = 0;
-        cstart.push_back(o);
-        ccase.push_back((int)ci);
-        int c0, r0;
-        band_cell(keys[o], gx, gy, c0, r0);
-        int r1 = r0, c1 = c0;
+        int c0 = (int)(cells[o] & 0xFFFFu), r0 = (int)(cells[o] >> 16);
+        int c1 = c0, r1 = r0;
         for (int i = 1; i < n; ++i) {
-            const int ki = keys[(size_t)o + i];
-            int cc, r;
-            band_cell(ki, gx, gy, cc, r);
+            const unsigned ki = cells[(size_t)o + i];
+            const int cc = (int)(ki & 0xFFFFu), r = (int)(ki >> 16);
             const int nr0 = std::min(r0, r), nr1 = std::max(r1, r), nc0 = std::min(c0, cc), nc1 = std::max(c1, cc);
-            const bool brk = (i - cs >= cap) || nr1 - nr0 > 1 || nc1 - nc0 > 125 || ki < 0 || keys[(size_t)o + cs] < 0;
-            if (brk) {
-                cstart.push_back(o + i);
-                ccase.push_back((int)ci);
+            if ((i - cs >= cap) || nr1 - nr0 > 1 || nc1 - nc0 > 125) {
+                st.push_back(o + i);
                 cs = i;
                 r0 = r1 = r;
                 c0 = c1 = cc;
@@ -1355,15 +1386,36 @@ void chunk_table(const icempm_batch *b, const std::vector<int> &keys, std::vecto
                 r0 = nr0; r1 = nr1; c0 = nc0; c1 = nc1;
             }
         }
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nt = std::min<size_t>(nc, std::min(16u, std::max(1u, hw)));
+    if (nt <= 1 || b->nPts < (1 << 16)) {
+        for (size_t ci = 0; ci < nc; ++ci) cut(ci);
+    } else {
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nt; ++t)
+            pool.emplace_back([&] {
+                for (size_t ci; (ci = next.fetch_add(1)) < nc;) cut(ci);
+            });
+        for (auto &t : pool) t.join();
     }
+    cstart.clear();
+    ccase.clear();
+    for (size_t ci = 0; ci < nc; ++ci)
+        for (int st : starts[ci]) {
+            cstart.push_back(st);
+            ccase.push_back((int)ci);
+        }
     cstart.push_back(b->nPts);
 }
 
 void drop_graph(icempm_batch *b) {
-    for (auto &g : b->graphs) {
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        g = icempm_batch::GraphSlot{};
-    }
+    for (auto &row : b->graphs)
+        for (auto &g : row) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = icempm_batch::GraphSlot{};
+        }
 }
 
 // the host chunk table to the device (grows the device array when needed)
@@ -1387,9 +1439,9 @@ int upload_chunks(icempm_batch *b) {
     return ICEMPM_OK;
 }
 
-// the case-local cell keys of the current slots (and, for a sort, the
-// offset keys and slot indices in sKeys / sVals)
-int current_keys(icempm_batch *b, std::vector<int> &local) {
+// the sort keys (batch-wide band keys) and slot indices of the current
+// slots in sKeys / sVals
+int sort_keys(icempm_batch *b) {
     const int n = b->nPts;
     if (!b->sKeys) {
         BCHK(b, hipMalloc(&b->sKeys, (size_t)n * sizeof(int)), "keys");
@@ -1404,13 +1456,27 @@ int current_keys(icempm_batch *b, std::vector<int> &local) {
                            static_cast<const double *>(b->pos), b->pitch, n, b->cases, (int)b->hc.size(), b->sKeys,
                            b->sVals);
     BCHK(b, hipGetLastError(), "keys");
-    local.resize((size_t)n);
-    BCHK(b, hipMemcpyAsync(local.data(), b->sKeys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->stream), "keys");
-    BCHK(b, hipStreamSynchronize(b->stream), "keys");
-    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
-        const CaseParams &cp = b->hc[ci].cp;
-        for (int i = 0; i < cp.nPts; ++i) local[(size_t)cp.ptOffset + i] -= cp.gridOffset;
+    return ICEMPM_OK;
+}
+
+// the base cells of the current slots, in b->hcells on the host
+int current_cells(icempm_batch *b) {
+    const int n = b->nPts;
+    if (!b->dcells) {
+        BCHK(b, hipMalloc(&b->dcells, (size_t)n * sizeof(unsigned)), "cells");
+        BCHK(b, hipHostMalloc(&b->hcells, (size_t)n * sizeof(unsigned), hipHostMallocDefault), "cells");
     }
+    if (b->precision == ICEMPM_BATCH_F32)
+        hipLaunchKernelGGL(k_batch_cells<unsigned int>, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                           static_cast<const unsigned int *>(b->pos), b->pitch, n, b->cases, (int)b->hc.size(),
+                           b->dcells);
+    else
+        hipLaunchKernelGGL(k_batch_cells<double>, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                           static_cast<const double *>(b->pos), b->pitch, n, b->cases, (int)b->hc.size(), b->dcells);
+    BCHK(b, hipGetLastError(), "cells");
+    BCHK(b, hipMemcpyAsync(b->hcells, b->dcells, (size_t)n * sizeof(unsigned), hipMemcpyDeviceToHost, b->stream),
+         "cells");
+    BCHK(b, hipStreamSynchronize(b->stream), "cells");
     return ICEMPM_OK;
 }
 
@@ -1437,8 +1503,7 @@ size_t rest_elem(const icempm_batch *b, int r, size_t slot) {
 int resort(icempm_batch *b) {
     const int n = b->nPts;
     if (n <= 1) return ICEMPM_OK;
-    std::vector<int> local;
-    int rc = current_keys(b, local);
+    int rc = sort_keys(b);
     if (rc) return rc;
     if (!b->sKeys2) {
         BCHK(b, hipMalloc(&b->sKeys2, (size_t)n * sizeof(int)), "re-sort");
@@ -1473,23 +1538,17 @@ int resort(icempm_batch *b) {
     std::swap(b->orig, b->torig);
     drop_graph(b);   // the captured launches hold the old row addresses
     b->pending = -1;   // a P2G in a grid scattered the points from their old slots
-    // the sorted keys give the new chunk table
-    BCHK(b, hipMemcpyAsync(local.data(), b->sKeys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->stream), "re-sort");
-    BCHK(b, hipStreamSynchronize(b->stream), "re-sort");
-    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
-        const CaseParams &cp = b->hc[ci].cp;
-        for (int i = 0; i < cp.nPts; ++i) local[(size_t)cp.ptOffset + i] -= cp.gridOffset;
-    }
-    chunk_table(b, local, b->hcstart, b->hccase);
+    // the cells of the sorted slots give the new chunk table
+    if ((rc = current_cells(b))) return rc;
+    chunk_table(b, b->hcells, b->hcstart, b->hccase);
     return upload_chunks(b);
 }
 
 // the chunk table for the current slots and chunkPoints
 int rebuild_chunks(icempm_batch *b) {
-    std::vector<int> local;
-    int rc = current_keys(b, local);
+    int rc = current_cells(b);
     if (rc) return rc;
-    chunk_table(b, local, b->hcstart, b->hccase);
+    chunk_table(b, b->hcells, b->hcstart, b->hccase);
     return upload_chunks(b);
 }
 
@@ -1587,6 +1646,9 @@ void icempm_batch_destroy(icempm_batch *b) {
     dfree(b->err); dfree(b->tTab); dfree(b->hist);
     dfree(b->mlpW); dfree(b->mlpP); dfree(b->hx); dfree(b->ht); dfree(b->hcount); dfree(b->stats);
     dfree(b->mapQ); dfree(b->mapCount);
+    dfree(b->dcells);
+    if (b->hcells) (void)hipHostFree(b->hcells);
+    b->hcells = nullptr;
     dfree(b->orig); dfree(b->sKeys); dfree(b->sVals); dfree(b->sKeys2); dfree(b->sVals2); dfree(b->sTmp);
     dfree(b->tpos); dfree(b->trest); dfree(b->tcrushed); dfree(b->torig);
     dfree(b->kpos); dfree(b->krest); dfree(b->kcrushed); dfree(b->korig);
@@ -1708,13 +1770,30 @@ int icempm_batch_finalize(icempm_batch *b) {
     // the particle pass addresses rows through 32-bit byte offsets
     if (pofs >= (1L << 29) || gofs >= (1L << 29) || bofs > INT_MAX / 2)
         return bfail(b, ICEMPM_E_ARG, "finalize: batch too large (2^29 points or grid nodes at most)");
+    // a base cell packs its column and row into 16 bits each
+    for (const auto &h : b->hc)
+        if (h.cp.gx > 65536 || h.cp.gy > 65536)
+            return bfail(b, ICEMPM_E_ARG, "finalize: a case's grid is wider or taller than 65536 nodes");
     b->binsTotal = (int)bofs;
     b->nPts = (int)pofs;
     b->nNodes = gofs;
     b->nNodeBlocks = (int)nblk.size();
     b->pitch = (pofs + 63) / 64 * 64;
     b->gpitch = ((size_t)gofs + 63) / 64 * 64;
-    chunk_table(b, keys, b->hcstart, b->hccase);
+    {
+        // the cells of the band keys, case-local (the key of a cell is
+        // band_key of its column and row)
+        std::vector<unsigned> cells(keys.size());
+        for (size_t ci = 0; ci < b->hc.size(); ++ci) {
+            const CaseParams &cp = b->hc[ci].cp;
+            for (int k = 0; k < cp.nPts; ++k) {
+                int cx, cy;
+                band_cell(keys[(size_t)cp.ptOffset + k], cp.gx, cp.gy, cx, cy);
+                cells[(size_t)cp.ptOffset + k] = (unsigned)cx | (unsigned)cy << 16;
+            }
+        }
+        chunk_table(b, cells.data(), b->hcstart, b->hccase);
+    }
     const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
     BCHK(b, hipMalloc(&b->pos, pos_bytes(b)), "batch finalize");
     BCHK(b, hipMalloc(&b->rest, 11 * b->pitch * vs), "batch finalize");
@@ -1815,9 +1894,16 @@ namespace {
 // The executable graph of n substeps from the given start (-1 fresh, 0 or 1:
 // the P2G in grid A or B) with these time-table and history rows, captured,
 // instantiated and put on the device unless its slot already holds it.
-int ensure_graph(icempm_batch *b, int n, const double *tTab, double *hist, int start) {
-    icempm_batch::GraphSlot &g = b->graphs[start + 1];
-    if (g.exec && g.steps == n && g.T == tTab && g.H == hist) return ICEMPM_OK;
+int ensure_graph(icempm_batch *b, int n, const double *tTab, double *hist, int start,
+                 const icempm_batch::GraphSlot **out) {
+    icempm_batch::GraphSlot *row = b->graphs[start + 1];
+    for (int w = 0; w < 2; ++w)
+        if (row[w].exec && row[w].steps == n && row[w].T == tTab && row[w].H == hist) {
+            row[w].used = ++b->graphTick;
+            if (out) *out = &row[w];
+            return ICEMPM_OK;
+        }
+    icempm_batch::GraphSlot &g = row[row[0].used <= row[1].used ? 0 : 1];
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     g = icempm_batch::GraphSlot{};
     hipGraph_t graph = nullptr;
@@ -1837,6 +1923,8 @@ int ensure_graph(icempm_batch *b, int n, const double *tTab, double *hist, int s
     g.T = tTab;
     g.H = hist;
     g.end = end;
+    g.used = ++b->graphTick;
+    if (out) *out = &g;
     // put the executable graph on the device now, so that its first launch
     // costs what a replay costs
     BCHK(b, hipGraphUpload(g.exec, b->stream), "batch graph upload");
@@ -1863,13 +1951,13 @@ int run_segment(icempm_batch *b, int n, int use_graph, const double *tTab, doubl
     b->pending = -1;   // until the loop below is enqueued whole
     int end = -1;
     if (use_graph) {
-        const int rc = ensure_graph(b, n, tTab, hist, start);
+        const icempm_batch::GraphSlot *g = nullptr;
+        const int rc = ensure_graph(b, n, tTab, hist, start, &g);
         if (rc) return rc;
-        const icempm_batch::GraphSlot &g = b->graphs[start + 1];
-        end = g.end;
+        end = g->end;
         // timed from here: capture and instantiation are host work before it
         BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
-        BCHK(b, hipGraphLaunch(g.exec, b->stream), "batch graph launch");
+        BCHK(b, hipGraphLaunch(g->exec, b->stream), "batch graph launch");
     } else {
         BCHK(b, hipEventRecord(b->e0, b->stream), "batch run");
         BCHK(b, enqueue_loop(b, n, tTab, hist, start, &end), "batch run");
@@ -1946,7 +2034,7 @@ int icempm_batch_prepare(icempm_batch *b, int n) {
     // the first segment of such a run, from each of the three starts
     const int len = b->resortEvery > 0 ? std::min(b->resortEvery, n) : n;
     for (int start = -1; start <= 1; ++start)
-        if ((rc = ensure_graph(b, len, b->tTab, b->hist, start))) return rc;
+        if ((rc = ensure_graph(b, len, b->tTab, b->hist, start, nullptr))) return rc;
     return ICEMPM_OK;
 }
 
@@ -2311,8 +2399,9 @@ int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int 
     if ((rc = icempm_batch_restore(b))) return rc;
     BCHK(b, hipMemcpy(b->stats, stats0, sizeof(stats0), hipMemcpyHostToDevice), "autotune");
     BCHK(b, hipMemcpy(b->err, &err0, sizeof(int), hipMemcpyHostToDevice), "autotune");
-    b->chunkPoints = bestChunk;
     b->resortEvery = bestResort;
+    if (bestChunk == b->chunkPoints) return ICEMPM_OK;   // the restored table is the one
+    b->chunkPoints = bestChunk;
     return rebuild_chunks(b);
 }
 

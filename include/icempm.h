@@ -243,7 +243,8 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
                           size_t pitch);
 int icempm_batch_finalize(icempm_batch *b);
 /* n substeps of every case from the batch's current time; use_graph != 0
- * captures the loop once per n (and start, below) and replays it.  The last
+ * captures the loop once per n (and start, below) and replays it, keeping
+ * the graphs of the two most recently used step counts per start.  The last
  * pass of a run leaves the next substep's P2G in the batch's grids, made
  * from the points as the run leaves them, and the next run starts from it;
  * after icempm_batch_restore or a re-sort the next run makes its own. */
@@ -252,8 +253,9 @@ int icempm_batch_run(icempm_batch *b, int n, int use_graph);
  * history buffers, and the graphs of such a run from each of its possible
  * starts (fresh, or continuing a previous run), captured, instantiated and
  * uploaded.  A run of n substeps then launches at once, whatever runs come
- * between (shorter runs keep the buffers; a run of another length from the
- * same start replaces that start's graph). */
+ * between (shorter runs keep the buffers; runs of two other lengths from the
+ * same start replace that start's graph).  Preparing the longer of two run
+ * lengths first keeps both. */
 int icempm_batch_prepare(icempm_batch *b, int n);
 /* device time of the last icempm_batch_run (HIP events on the batch stream;
  * a graph run is timed from its launch, after capture, instantiation and
