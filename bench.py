@@ -473,7 +473,7 @@ def run_batch_learned(args) -> dict:
             "cases": len(cases),
             "points": n_total,
             "grid_nodes": nodes,
-            "storage": "f32 (positions 48-bit fixed point over each grid width)",
+            "storage": "f32 (positions 48-bit fixed point in cell units)",
             "mlp": "bf16 weights/activations, f32 accumulation",
             "points_per_chunk": chunk,
             "resort_every": resort_every,

@@ -47,7 +47,10 @@ struct CaseParams {
     double ics, its, iss, tanPhi, dpThr, x0, y0;
     double pmax, msq, betaPmax, onePlus2Beta, slope;   // NACC constants of the case's material (grain 0)
     double grainVar;        // GrainVariability: 0 = the constants above hold for every point
-    double posQ, posInvQ;   // FP32 storage: position quantum (grid width / 2^48) and its inverse
+    // FP32 storage: a coordinate is a 48-bit integer u with x / h = u 2^-F,
+    // F = 48 - the bits a cell index of the case needs: the quantum in metres
+    // (h 2^-F) and its inverse, and in cell units 2^-F, 2^(16 - F), 2^F
+    double posQ, posInvQ, cellQ, cellQ16, cellInvQ;
     double scm;             // dt V_p D^-1 / m_p: the stress term of the affine momentum per unit mass
     int gx, gy, gridTotal, nPts, ptOffset, gridOffset;
     int nbins, binOffset;   // indenter force bins: n_indenter_subdivisions, first double in the bin array
@@ -56,11 +59,14 @@ struct CaseParams {
 constexpr int kWX = 48, kWS = 52, kWY = 4;   // window width, padded row stride, rows
 
 // Point rows.  FP64 storage: positions FP64.  FP32 storage: positions as
-// 48-bit fixed point over the case's grid width (a quantum of gx h / 2^48,
-// ~1e-14 m for a 3 m grid: a substep moves a point by ~1e-6 m, which an FP32
-// coordinate could not resolve, and 32 bits -- a quantum of ~7e-10 m --
-// rounded each displacement by ~1e-4 of itself, enough to move the force
-// history of a fracturing case by ~1%), the other values FP32.
+// 48-bit fixed point in units of h 2^-F, the cell index in the top 48 - F
+// bits (F = 48 - the bits of the larger grid dimension: a quantum of at most
+// ~2 gx h / 2^48, ~2e-14 m for a 3 m grid; a substep moves a point by ~1e-6
+// m, which an FP32 coordinate could not resolve, and 32 bits -- a quantum of
+// ~7e-10 m -- rounded each displacement by ~1e-4 of itself, enough to move
+// the force history of a fracturing case by ~1%); the base cell and in-cell
+// offset of a stencil are exact functions of the integer.  The other values
+// FP32.
 template <bool F32>
 struct Rows {
     using V = typename std::conditional<F32, float, double>::type;
@@ -93,10 +99,10 @@ static __device__ __forceinline__ T *at(T *row, unsigned i) {
     return reinterpret_cast<T *>(reinterpret_cast<B *>(row) + i * (unsigned)sizeof(T));
 }
 
-// FP32 storage: a coordinate as 48-bit fixed point, u = hi 2^16 + lo
-// (round to nearest; outside [0, gx h) and NaN go to the grid's edge, where
-// the next stencil check flags them).  Every step is exact in FP64 except
-// the scalings by posQ / posInvQ.
+// FP32 storage: a coordinate as 48-bit fixed point, u = hi 2^16 + lo, in
+// metres (round to nearest; below 0 to 0, above 2^48 - 1 quanta and NaN to
+// 2^48 - 1, past the grid's edge, where the next stencil check flags them).
+// Every step is exact in FP64 except the scalings by posQ / posInvQ.
 static __host__ __device__ inline double fix48_get(unsigned int hi, unsigned int lo, const CaseParams &cp) {
     return ((double)hi * 65536.0 + (double)lo) * cp.posQ;
 }
@@ -107,10 +113,33 @@ static __host__ __device__ inline void fix48_put(double x, const CaseParams &cp,
     hi = (unsigned int)uh;
     lo = (unsigned int)(u - uh * 65536.0);
 }
-// whether a coordinate lies where 48-bit fixed point represents it
-static __host__ __device__ inline bool fix48_in_range(double x, const CaseParams &cp) {
-    const double u = x * cp.posInvQ;
-    return u >= 0.0 && u < 281474976710656.0;
+// The same in cell units (xr = x / h = u 2^-F): exact both ways, the
+// scalings being powers of 2.  fix_put_cells returns u.
+static __device__ __forceinline__ double fix_cells(unsigned int hi, unsigned int lo, const CaseParams &cp) {
+    return fma((double)hi, cp.cellQ16, (double)lo * cp.cellQ);
+}
+static __device__ __forceinline__ double fix_put_cells(double xr, const CaseParams &cp, unsigned int &hi,
+                                                       unsigned int &lo) {
+    double u = rint(xr * cp.cellInvQ);
+    u = (u >= 0.0 && u <= kFix48Max) ? u : (u < 0.0 ? 0.0 : kFix48Max);
+    const double uh = floor(u * (1.0 / 65536.0));
+    hi = (unsigned int)uh;
+    lo = (unsigned int)(u - uh * 65536.0);
+    return u;
+}
+// The quadratic B-spline stencil (gpu_implementation5.cu:185-193) of a
+// coordinate in cell units: base cell floor(xr - 0.5), 0 below xr = 0.5 (as
+// the truncation of the FP64 form gives), and offset fx = xr - base, both
+// exact for the xr of a record.
+static __device__ __forceinline__ void stencil_cells(double xr, int &base, double &fx, double &w0, double &w1,
+                                                     double &w2) {
+    const double b = fmax(floor(xr - 0.5), 0.0);
+    base = (int)b;
+    fx = xr - b;
+    const double v0 = 1.5 - fx, v1 = fx - 1.0, v2 = fx - 0.5;
+    w0 = 0.5 * v0 * v0;
+    w1 = 0.75 - v1 * v1;
+    w2 = 0.5 * v2 * v2;
 }
 
 // Point position of slot pt (both storage formats) as FP64; store it back,
@@ -128,8 +157,8 @@ static __device__ __forceinline__ void pos_load(const double *pos, size_t pp, in
     }
 }
 typedef unsigned int prec4 __attribute__((ext_vector_type(4)));
-// FP32 storage: the slot's record; returns its last word (crushed flag in
-// bits 0-7, grain id in bits 16-31)
+// FP32 storage: the slot's record, its coordinates in metres; returns its
+// last word (crushed flag in bits 0-7, grain id in bits 16-31)
 template <bool NT>
 static __device__ __forceinline__ unsigned pos_load(const unsigned int *pos, size_t, int pt, const CaseParams &cp,
                                                     double &x, double &y) {
@@ -143,34 +172,14 @@ static __device__ __forceinline__ unsigned pos_load(const unsigned int *pos, siz
     y = fix48_get(v.y, v.z >> 16, cp);
     return v.w;
 }
-// stores (x, y) and replaces them by the values the rows now hold.  FP64
-// rows are stored through the L2 (a wavefront's 512 B of a row straddle
-// cache lines that the neighbouring wavefronts complete: measured 6% faster
-// than non-temporal stores); the FP32 rows' 256 B stream non-temporally
-// (measured faster that way)
-static __device__ __forceinline__ void pos_store(double *pos, size_t pp, int pt, const CaseParams &, double &x,
-                                                 double &y, unsigned) {
+// FP64 storage: stores (x, y), through the L2 (a wavefront's 512 B of a row
+// straddle cache lines that the neighbouring wavefronts complete: measured
+// 6% faster than non-temporal stores).  The FP32 record streams
+// non-temporally (measured faster that way).
+static __device__ __forceinline__ void pos_store(double *pos, size_t pp, int pt, double x, double y) {
     const unsigned u = (unsigned)pt;
     *at(pos, u) = x;
     *at(pos + pp, u) = y;
-}
-// FP32 storage: the whole record, its last word w as pos_load returns it
-static __device__ __forceinline__ void pos_store(unsigned int *pos, size_t, int pt, const CaseParams &cp,
-                                                 double &x, double &y, unsigned w) {
-    unsigned int hx, lx, hy, ly;
-    fix48_put(x, cp, hx, lx);
-    fix48_put(y, cp, hy, ly);
-    const prec4 v = {hx, hy, lx | (ly << 16), w};
-    __builtin_nontemporal_store(v, reinterpret_cast<prec4 *>(pos) + pt);
-    x = fix48_get(hx, lx, cp);
-    y = fix48_get(hy, ly, cp);
-}
-// the value a coordinate takes when stored (without storing it)
-static __host__ __device__ inline double pos_round(double x, const CaseParams &, double) { return x; }
-static __host__ __device__ inline double pos_round(double x, const CaseParams &cp, unsigned int) {
-    unsigned int h, l;
-    fix48_put(x, cp, h, l);
-    return fix48_get(h, l, cp);
 }
 
 // FP32 storage keeps a point's Fe - I as one 16-byte record in the space of
@@ -369,9 +378,12 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             // streamed once per substep: non-temporal, so the grid keeps the L2
             const unsigned u = (unsigned)pt;
             if constexpr (F32) {
-                const unsigned w = pos_load<true>(S.pos, pp, pt, cp, px, py);
-                crushed = (unsigned char)(w & 0xFFu);
-                grain = (short)(w >> 16);
+                // the record; its coordinates in cell units
+                const prec4 r = __builtin_nontemporal_load(reinterpret_cast<const prec4 *>(S.pos) + pt);
+                px = fix_cells(r.x, r.z & 0xFFFFu, cp);
+                py = fix_cells(r.y, r.z >> 16, cp);
+                crushed = (unsigned char)(r.w & 0xFFu);
+                grain = (short)(r.w >> 16);
                 const fe4 f = __builtin_nontemporal_load(fe_rec(R, pp) + pt);
                 F00 = fe0 + (double)f.x;
                 F01 = (double)f.y;
@@ -389,9 +401,18 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         }
         int bx = 0, by = 0;
         double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
-        const bool ok = valid && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
-                        stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
-        if (valid && !ok) flag_error(k);
+        bool ok;
+        if constexpr (F32) {   // cell units: a record always has a stencil
+            ok = valid;
+            if (ok) {
+                stencil_cells(px, bx, fx0, wx0, wx1, wx2);
+                stencil_cells(py, by, fx1, wy0, wy1, wy2);
+            }
+        } else {
+            ok = valid && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
+                 stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
+            if (valid && !ok) flag_error(k);
+        }
         InvSq iv{};
         unsigned char crushed0 = crushed;
         double Jp0 = Jp;
@@ -467,8 +488,9 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             }
             b00 *= c4; b01 *= c4; b10 *= c4; b11 *= c4;
             const double dt = cp.dt;
-            px += v0 * dt;
-            py += v1 * dt;
+            const double dtp = F32 ? dt * cp.h_inv : dt;   // FP32 storage: cell units
+            px += v0 * dtp;
+            py += v1 * dtp;
             const double M00 = 1.0 + dt * b00, M01 = dt * b01, M10 = dt * b10, M11 = 1.0 + dt * b11;
             const double n00 = M00 * F00 + M01 * F10, n01 = M00 * F01 + M01 * F11;
             const double n10 = M10 * F00 + M11 * F10, n11 = M10 * F01 + M11 * F11;
@@ -566,9 +588,14 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                         e.F[0] = F00; e.F[1] = F01; e.F[2] = F10; e.F[3] = F11;
                         e.Jp = Jp;
                         // the position at its stored precision, as the P2G here would take it
-                        using PT = typename Rows<F32>::P;
-                        e.x[0] = pos_round(px, cp, PT{});
-                        e.x[1] = pos_round(py, cp, PT{});
+                        if constexpr (F32) {
+                            unsigned int hi, lo;
+                            e.x[0] = fix_put_cells(px, cp, hi, lo) * cp.posQ;
+                            e.x[1] = fix_put_cells(py, cp, hi, lo) * cp.posQ;
+                        } else {
+                            e.x[0] = px;
+                            e.x[1] = py;
+                        }
                         e.v[0] = vx; e.v[1] = vy;
                         e.B[0] = B00; e.B[1] = B01; e.B[2] = B10; e.B[3] = B11;
                     } else {
@@ -589,14 +616,22 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         rt = iv;
         fresh = ok && crushed == 0;
         if (ok) {
-            if constexpr (F32) {
-                // fixed point clamps to [0, gx h): a position outside (or
-                // NaN) is an error here, as the FP64 rows would report it
-                if (!(fix48_in_range(px, cp) && fix48_in_range(py, cp))) flag_error(k);
-            }
             // px, py become what the rows hold: the P2G takes them at their
             // stored precision
-            pos_store(S.pos, pp, pt, cp, px, py, (unsigned)crushed | ((unsigned)(unsigned short)grain << 16));
+            if constexpr (F32) {
+                // a position outside the grid (or NaN) is an error here, as
+                // the next stencil of FP64 rows would report it; the fixed
+                // point clamps it
+                if (!(px >= 0.0 && px < (double)gx && py >= 0.0 && py < (double)gy)) flag_error(k);
+                unsigned int hx, lx, hy, ly;
+                px = fix_put_cells(px, cp, hx, lx) * cp.cellQ;
+                py = fix_put_cells(py, cp, hy, ly) * cp.cellQ;
+                const prec4 r = {hx, hy, lx | (ly << 16),
+                                 (unsigned)crushed | ((unsigned)(unsigned short)grain << 16)};
+                __builtin_nontemporal_store(r, reinterpret_cast<prec4 *>(S.pos) + pt);
+            } else {
+                pos_store(S.pos, pp, pt, px, py);
+            }
             const unsigned u = (unsigned)pt;
             if (!defer) {
                 if constexpr (F32) {
@@ -639,7 +674,13 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
         bool have = valid;
         if constexpr (!G2P) {
             if (valid) {
-                pos_load<false>(S.pos, pp, pt, cp, px, py);
+                if constexpr (F32) {
+                    const prec4 r = reinterpret_cast<const prec4 *>(S.pos)[pt];
+                    px = fix_cells(r.x, r.z & 0xFFFFu, cp);
+                    py = fix_cells(r.y, r.z >> 16, cp);
+                } else {
+                    pos_load<false>(S.pos, pp, pt, cp, px, py);
+                }
                 vx = (double)R[pt];
                 vy = (double)R[pp + pt];
                 if constexpr (F32) {
@@ -660,12 +701,21 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                 B11 = (double)R[pp * 9 + pt];
             }
         } else {
-            have = valid && !defer && !(px != px || py != py);
+            have = valid && !defer && (F32 || !(px != px || py != py));
         }
         int bx = 0, by = 0;
         double fx0 = 0, fx1 = 0, wx0, wx1, wx2, wy0, wy1, wy2;
-        const bool ok = have && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
-                        stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
+        bool ok;
+        if constexpr (F32) {   // cell units
+            ok = have;
+            if (ok) {
+                stencil_cells(px, bx, fx0, wx0, wx1, wx2);
+                stencil_cells(py, by, fx1, wy0, wy1, wy2);
+            }
+        } else {
+            ok = have && stencil(px, cp.h_inv, bx, fx0, wx0, wx1, wx2) &&
+                 stencil(py, cp.h_inv, by, fx1, wy0, wy1, wy2);
+        }
         if (valid && !ok && !defer) flag_error(k);   // deferred points scatter in the map kernel
         // the box of base cells of the lanes whose stencils can reach the
         // window (a lane outside the grid touches no node): the window's
@@ -1710,8 +1760,18 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
     c.nbins = P.n_indenter_subdivisions;
     c.binOffset = 0;   // set by finalize
     c.scm = c.dt * c.vol * c.Dinv / c.mass;
-    c.posQ = (c.gx * c.h) / 281474976710656.0;   // FP32 storage: positions in [0, gx h) on 48 bits
-    c.posInvQ = 1.0 / c.posQ;
+    {
+        // FP32 storage: 48 - ib fraction bits, ib the bits a cell index needs
+        // (finalize refuses grids beyond 65536 nodes a side: ib <= 16)
+        int ib = 0;
+        while (ib < 16 && (1 << ib) < std::max(c.gx, c.gy)) ++ib;
+        const int F = 48 - ib;
+        c.cellQ = std::ldexp(1.0, -F);
+        c.cellQ16 = std::ldexp(1.0, 16 - F);
+        c.cellInvQ = std::ldexp(1.0, F);
+        c.posQ = c.h * c.cellQ;
+        c.posInvQ = c.cellInvQ / c.h;
+    }
     const int n = P.nPts;
     h.pos.resize(2 * (size_t)n);
     h.rest.resize(11 * (size_t)n);

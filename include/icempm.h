@@ -226,8 +226,9 @@ int icempm_read_bins(icempm_ctx *ctx, double *out);
  * gpu_implementation5.cu:97-111); otherwise its material is uniform.
  * Precision: ICEMPM_BATCH_F64 stores the point state in FP64;
  * ICEMPM_BATCH_F32 stores velocity, Fe - I, Bp and Jp_inv - 1 in FP32 and the
- * positions as 48-bit fixed point over the case's grid width (a quantum of
- * GridX * cellsize / 2^48), each point's position (with its crushed flag) and
+ * positions as 48-bit fixed point in units of cellsize * 2^-F, F = 48 - the
+ * bits of max(GridX, GridY) (grids up to 65536 nodes a side; a quantum of at
+ * most 2 * GridX * cellsize / 2^48), each point's position (with its crushed flag) and
  * its Fe - I as one 16-byte record apiece; the grid sums stay FP64, the grid velocities
  * reach the gathers as FP32 pairs, and all arithmetic is FP64.  The
  * per-substep total indenter force of every case is recorded, and its

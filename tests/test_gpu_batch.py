@@ -31,9 +31,10 @@ def oracle_case(p0, soa0, nsub):
 
 
 # Positions: FP64 storage keeps them to 1e-12 m.  FP32 storage holds them as
-# 48-bit fixed point over the grid width (quantum gx h / 2^48, ~1e-14 m on
-# these grids) and Fe - I in FP32 (~6e-8 of itself, which moves the stress
-# by as much): 1e-10 m over 150 substeps (1e-8 of a cell).
+# 48-bit fixed point in units of h 2^-F, F = 48 - the bits of the larger grid
+# dimension (quantum ~1e-14 m on these grids), and Fe - I in FP32 (~6e-8 of
+# itself, which moves the stress by as much): 1e-10 m over 150 substeps
+# (1e-8 of a cell).
 F32_TOL_X = 1e-10
 
 
@@ -195,8 +196,10 @@ def test_batch_state_round_trip(gpu_available, precision):
                 assert np.array_equal(dev[r, :n], soa0[r, :n]), r
         else:
             # half a quantum, the rounding of x / q (< 2^-5 of a quantum
-            # for u < 2^48) and of the final scaling by q
-            q = p.GridX * p.cellsize / 2.0 ** 48
+            # for u < 2^48) and of the final scaling by q; the quantum is
+            # h 2^-F with 48 - F the bits of the larger grid dimension
+            ib = int(np.ceil(np.log2(max(p.GridX, p.GridY))))
+            q = p.cellsize * 2.0 ** -(48 - ib)
             bound = (0.5 + 2.0 ** -5) * q + np.spacing(np.abs(soa0[4:6, :n]))
             assert (np.abs(dev[4:6, :n] - soa0[4:6, :n]) <= bound).all()
             for r in list(range(6, 16)) + [3]:

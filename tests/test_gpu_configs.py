@@ -46,9 +46,9 @@ def test_plain_at_scale_fp32_storage(gpu_available):
     """1.6M points, 300 substeps.  FP64 batch storage follows the single-case
     engine to the rounding of the sums' order; FP32 storage keeps the
     indenter force within 1e-3 (relative L2) and the positions within 1e-9 m
-    (2e-7 of a cell; the 48-bit position quantum on this 6 m grid is
-    2e-14 m, and the FP32 rounding of Fe - I moves the stress by ~6e-8 of
-    itself)."""
+    (2e-7 of a cell; the 48-bit position quantum on this 1200-node grid
+    is h 2^-37, 4e-14 m, and the FP32 rounding of Fe - I moves the stress by
+    ~6e-8 of itself)."""
     case, nsub = INDENT_1M6, 300
     print("plain at scale", case.name, flush=True)
     p0, soa0, h64, x64 = single_case_fp64(case, nsub)
