@@ -51,6 +51,7 @@ struct CaseParams {
     // F = 48 - the bits a cell index of the case needs: the quantum in metres
     // (h 2^-F) and its inverse, and in cell units 2^-F, 2^(16 - F), 2^F
     double posQ, posInvQ, cellQ, cellQ16, cellInvQ;
+    double gxInv;           // 1 / gx: a node's row from its index
     double scm;             // dt V_p D^-1 / m_p: the stress term of the affine momentum per unit mass
     int gx, gy, gridTotal, nPts, ptOffset, gridOffset;
     int nbins, binOffset;   // indenter force bins: n_indenter_subdivisions, first double in the bin array
@@ -996,10 +997,14 @@ __global__ __launch_bounds__(256) void k_batch_map(Rows<F32> S, const CaseParams
 }
 
 // Grid update over all cases: block b handles nodes [nodeBlock[b].y, +256)
-// of case nodeBlock[b].x.  Same arithmetic as the single-case update; the
-// substep's total indenter force of each case goes to hist[case*2 + comp].
-// The velocities go to rows 1-2 of the grid, or, when vel32 is given (FP32
-// point storage), to vel32 as FP32 pairs, zero for the nodes without mass.
+// of case nodeBlock[b].x.  The single-case update (gpu_implementation5.cu:
+// 215-290) with three FAST forms: 1/m from rcp_nr, the speed limit tested on
+// the squared speed, the node's row from a reciprocal of the grid width.
+// The substep's total indenter force of each case goes to hist[case*2 +
+// comp].  The velocities go to rows 1-2 of the grid, or, when vel32 is given
+// (FP32 point storage), to vel32 as FP32 pairs, zero for the nodes without
+// mass.  A node's mass and momentum are loaded together, so that a block
+// waits for memory once.
 // Per-block flags skip the stores that would write zeros over zeros: a
 // block of zeroGrid is cleared only if it took mass in its last P2G
 // (zeroHasMass, recorded when that grid was last updated), and a block of
@@ -1017,7 +1022,16 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
     __shared__ double sf[2];
     __shared__ int any;
     const int2 nb = nodeBlock[blockIdx.x];
-    const CaseParams cp = cases[nb.x];
+    const CaseParams &cp = cases[nb.x];
+    const int local = nb.y + (int)threadIdx.x;
+    const bool node = local < cp.gridTotal;
+    const size_t idx = (size_t)cp.gridOffset + (node ? local : 0);
+    double mass = 0, p0 = 0, p1 = 0;
+    if (node) {
+        mass = grid[idx];
+        p0 = grid[gpitch + idx];
+        p1 = grid[2 * gpitch + idx];
+    }
     if (threadIdx.x == 0) {
         sf[0] = 0;
         sf[1] = 0;
@@ -1027,10 +1041,6 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
     // and a block of vel32 last written with no mass holds zeros
     const bool zeroIt = zeroGrid && zeroHasMass[blockIdx.x] != 0;
     const bool velDirty = vel32 && velNonzero[blockIdx.x] != 0;
-    const int local = nb.y + (int)threadIdx.x;
-    const bool node = local < cp.gridTotal;
-    const size_t idx = (size_t)cp.gridOffset + (node ? local : 0);
-    const double mass = node ? grid[idx] : 0.0;
     const bool blockMass = __syncthreads_or(mass != 0) != 0;   // also orders sf/any
     if (threadIdx.x == 0) {
         gridHasMass[blockIdx.x] = blockMass ? 1 : 0;
@@ -1047,15 +1057,21 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
             const double dt = cp.dt, h = cp.h, V = cp.indV;
             const double t = tTab[step];
             const double ind_x = cp.x0 + t * V, ind_y = cp.y0;
-            double v0 = grid[gpitch + idx] / mass, v1 = grid[2 * gpitch + idx] / mass;
+            const double rm = rcp_nr(mass);
+            double v0 = p0 * rm, v1 = p1 * rm;
             v1 = v1 - dt * cp.gravity;
             const double vmax = 0.5 * h / dt;
-            const double vn = sqrt(v0 * v0 + v1 * v1);
-            if (vn > vmax) {
-                v0 = v0 / vn * vmax;
-                v1 = v1 / vn * vmax;
+            const double vsq = v0 * v0 + v1 * v1;
+            if (vsq > vmax * vmax) {
+                const double s = vmax / sqrt(vsq);
+                v0 = v0 * s;
+                v1 = v1 * s;
             }
-            const int ix = local % cp.gx, iy = local / cp.gx;
+            // row and column: (local + 0.5) / gx lies >= 0.5 / gx >= 2^-17
+            // from an integer and is computed to within 2^-23, so its floor
+            // is the row
+            const int iy = (int)(((double)local + 0.5) * cp.gxInv);
+            const int ix = local - iy * cp.gx;
             double n0 = ix * h - ind_x, n1 = iy * h - ind_y;
             if (n0 * n0 + n1 * n1 < cp.indRsq) {
                 const double r0 = v0 - V, r1 = v1;
@@ -1067,10 +1083,10 @@ __global__ __launch_bounds__(256) void k_batch_update(const CaseParams *__restri
                 }
                 const double vnr = r0 * n0 + r1 * n1;
                 if (vnr < 0) {
-                    const double p0 = v0, p1 = v1;
+                    const double q0 = v0, q1 = v1;
                     v0 = V + (r0 - n0 * vnr);
                     v1 = r1 - n1 * vnr;
-                    const double f0 = (p0 - v0) * mass / dt, f1 = (p1 - v1) * mass / dt;
+                    const double f0 = (q0 - v0) * mass / dt, f1 = (q1 - v1) * mass / dt;
                     atomicAdd(&sf[0], f0);
                     atomicAdd(&sf[1], f1);
                     any = 1;
@@ -1756,6 +1772,7 @@ int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes,
         c.slope = -(2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin)) / -3e6;
     }
     c.gx = P.GridX; c.gy = P.GridY; c.gridTotal = P.GridTotal; c.nPts = P.nPts;
+    c.gxInv = 1.0 / c.gx;
     c.grainVar = P.GrainVariability;
     c.nbins = P.n_indenter_subdivisions;
     c.binOffset = 0;   // set by finalize
