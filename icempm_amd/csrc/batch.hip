@@ -333,6 +333,107 @@ constexpr int kWaveWinBytes = 3 * kWY * kWS * 8;
 // which the kernel receives preloaded in SGPRs: a workgroup's first load is
 // its chunk {first point, end, case}, the point loads follow it at once,
 // and the case's parameters arrive meanwhile.
+// Quad-level P2G.  A DPP read of a double from another lane of the quad
+// (quad_perm control CTRL; every lane of the quad must be active, so no
+// lane keeps an old value and none is needed)
+template <int CTRL>
+static __device__ __forceinline__ double quad_dpp(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kQuadXor3 = 0x1B;   // quad_perm 1032, 2301, 3210
+// whether the four lanes of this lane's quad hold the same key (all lanes
+// of the wavefront active)
+static __device__ __forceinline__ bool quad_uniform(unsigned key) {
+    const int k = (int)key;
+    return k == __builtin_amdgcn_mov_dpp(k, kQuadXor1, 0xF, 0xF, true) &&
+           k == __builtin_amdgcn_mov_dpp(k, kQuadXor2, 0xF, 0xF, true) &&
+           k == __builtin_amdgcn_mov_dpp(k, kQuadXor3, 0xF, 0xF, true);
+}
+// One component's ten node values of a quad lane (step 1 layout, below)
+// summed over the quad and added to the window row Wc.
+static __device__ __forceinline__ void quad_component(int lane, const double (&K)[5], const double (&D)[4], int eA,
+                                                      int eB, int eC, double *Wc) {
+    const int sy = (lane >> 1) & 1;
+    double k[5];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) k[s] = K[s] + quad_dpp<kQuadXor1>(D[s]);
+    k[4] = K[4] + quad_dpp<kQuadXor1>(K[4]);
+    // step 2: the partners across bit 1 split slots (0, 1) and (2, 3)
+    const double x01 = sy ? k[0] : k[1], y01 = sy ? k[1] : k[0];
+    const double x23 = sy ? k[2] : k[3], y23 = sy ? k[3] : k[2];
+    atomicAdd(&Wc[eA], y01 + quad_dpp<kQuadXor2>(x01));
+    atomicAdd(&Wc[eB], y23 + quad_dpp<kQuadXor2>(x23));
+    const double k4 = k[4] + quad_dpp<kQuadXor2>(k[4]);
+    if ((lane & 3) == 0) atomicAdd(&Wc[eC], k4);
+}
+// The P2G of a quad whose four points share their base cell, into the
+// window (e0: the base's element): node (i, j) of the stencil gets
+// W (v + A d), W = wx_i wy_j, d = ((i, j) - f) h, summed over the four
+// points.  Step 1 pairs lanes across lane bit 0 (sx): slot s < 3 holds node
+// (2 sx, s), kept, and (2 - 2 sx, s), sent to the partner; slot 3 node
+// (1, sx), kept, and (1, 1 - sx), sent; slot 4 node (1, 2) on both.  After
+// it a lane holds pair sums of five (sx = 0) or four (sx = 1) nodes.  Step 2
+// pairs lanes across bit 1 (sy): of slots 0 and 1 the lane keeps slot sy,
+// of slots 2 and 3 slot 2 + sy, and lane 0 of the quad keeps slot 4.  Each
+// lane then adds two or three of the nine node sums per row: 6-9 LDS
+// atomics instead of 27, the nodes of one instruction still distinct
+// within the quad.
+static __device__ __forceinline__ void quad_scatter(int lane, int e0, double h, double fx0, double fx1, double wx0,
+                                                    double wx1, double wx2, double wy0, double wy1, double wy2,
+                                                    double vx, double vy, double A00, double A01, double A10,
+                                                    double A11, double *W0, double *W1, double *W2) {
+    const int sx = lane & 1, sy = (lane >> 1) & 1;
+    const double dx0 = (0.0 - fx0) * h, dx1 = (1.0 - fx0) * h, dx2 = (2.0 - fx0) * h;
+    const double dy0 = (0.0 - fx1) * h, dy1 = (1.0 - fx1) * h, dy2 = (2.0 - fx1) * h;
+    const double xa0 = vx + A00 * dx0, xa1 = vx + A00 * dx1, xa2 = vx + A00 * dx2;
+    const double xb0 = vy + A10 * dx0, xb1 = vy + A10 * dx1, xb2 = vy + A10 * dx2;
+    const double ya0 = A01 * dy0, ya1 = A01 * dy1, ya2 = A01 * dy2;
+    const double yb0 = A11 * dy0, yb1 = A11 * dy1, yb2 = A11 * dy2;
+    // the kept and sent columns of slots 0-2, and the kept and sent rows of slot 3
+    const double wxK = sx ? wx2 : wx0, xaK = sx ? xa2 : xa0, xbK = sx ? xb2 : xb0;
+    const double wxD = sx ? wx0 : wx2, xaD = sx ? xa0 : xa2, xbD = sx ? xb0 : xb2;
+    const double wyK = sx ? wy1 : wy0, yaK = sx ? ya1 : ya0, ybK = sx ? yb1 : yb0;
+    const double wyD = sx ? wy0 : wy1, yaD = sx ? ya0 : ya1, ybD = sx ? yb0 : yb1;
+    const double wy[3] = {wy0, wy1, wy2}, ya[3] = {ya0, ya1, ya2}, yb[3] = {yb0, yb1, yb2};
+    double WK[5], WD[4];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        WK[s] = wxK * wy[s];
+        WD[s] = wxD * wy[s];
+    }
+    WK[3] = wx1 * wyK;
+    WD[3] = wx1 * wyD;
+    WK[4] = wx1 * wy2;
+    // the nodes a lane adds: slot sy's node (2 sx, sy); slot 2 + sy's node,
+    // (2 sx, 2) or (1, sx); slot 4's node (1, 2)
+    const int eA = e0 + 2 * sx + sy * kWS;
+    const int eB = e0 + (sy ? 1 + sx * kWS : 2 * sx + 2 * kWS);
+    const int eC = e0 + 1 + 2 * kWS;
+    quad_component(lane, WK, WD, eA, eB, eC, W0);
+    double K[5], D[4];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        K[s] = WK[s] * (xaK + ya[s]);
+        D[s] = WD[s] * (xaD + ya[s]);
+    }
+    K[3] = WK[3] * (xa1 + yaK);
+    D[3] = WD[3] * (xa1 + yaD);
+    K[4] = WK[4] * (xa1 + ya2);
+    quad_component(lane, K, D, eA, eB, eC, W1);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        K[s] = WK[s] * (xbK + yb[s]);
+        D[s] = WD[s] * (xbD + yb[s]);
+    }
+    K[3] = WK[3] * (xb1 + ybK);
+    D[3] = WD[3] * (xb1 + ybD);
+    K[4] = WK[4] * (xb1 + yb2);
+    quad_component(lane, K, D, eA, eB, eC, W2);
+}
+
 template <bool F32, bool G2P, bool P2G, int RMAP>
 __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict__ chunks, int nChunks,
                                                          const CaseParams *__restrict__ cases,
@@ -746,6 +847,13 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             }
         }
         wsync();
+        // A quad of lanes whose four points share one base cell (in band
+        // order most quads, until points cross into other cells) and whose
+        // stencil lies in the window sums its contributions in registers
+        // before they reach the window.  Every lane is active here.
+        const bool winFits = ok && bx + 2 < gx && by + 2 < gy && (unsigned)(bx - ox) < (unsigned)(kWX - 2) &&
+                             (unsigned)(by - oy) < (unsigned)(kWY - 2);
+        const bool quad = quad_uniform(winFits ? (unsigned)bx | (unsigned)by << 16 : 0xFFFF0000u | (unsigned)lane);
         if (ok) {
             double t00, t01, t11;
             if (fresh)
@@ -762,8 +870,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             // 0 <= lx0 <= kWX - 3 and 0 <= ly0 <= kWY - 3, one unsigned
             // compare each (the origin is the least base cell of the lanes
             // that get here, so lx0, ly0 >= 0 already)
-            if (bx + 2 < gx && by + 2 < gy && (unsigned)lx0 < (unsigned)(kWX - 2) &&
-                (unsigned)ly0 < (unsigned)(kWY - 2)) {
+            if (quad) {
+                quad_scatter(lane, ly0 * kWS + lx0, h, fx0, fx1, wx0, wx1, wx2, wy0, wy1, wy2, vx, vy, A00, A01, A10,
+                             A11, W0, W1, W2);
+            } else if (winFits) {
                 const int e0 = ly0 * kWS + lx0;
                 // The four lanes of a quad (in band order mostly the points of
                 // one cell) walk the 3 x 3 stencil from four different nodes,
