@@ -1321,6 +1321,19 @@ using namespace icempm::batch;
 struct icempm_batch {
     int precision = ICEMPM_BATCH_F64;
     hipStream_t stream = nullptr;
+    // The cases in two halves of about equal point counts run their substep
+    // loops as two independent branches (the second on stream2), so that one
+    // half's grid update, bound by memory, overlaps the other's particle
+    // pass, bound by issue.  grpChunk / grpBlock: the halves' ranges of the
+    // chunk table and of the node blocks, {0, split, end}.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    int grpChunk[3] = {0, 0, 0};
+    int grpBlock[3] = {0, 0, 0};
+    int splitCase = 0;   // first case of the second half (0: one branch)
+    // the launch target of the next particle pass: its stream and chunk range
+    hipStream_t lstream = nullptr;
+    int lchunk0 = 0, lchunks = 0;
     std::string lastError;
     bool finalized = false;
 
@@ -1466,14 +1479,14 @@ void launch_one(icempm_batch *b, unsigned nb, const void *gV, double *gO, const 
                 const MapQueue &mq) {
     Rows<F32> S{static_cast<typename Rows<F32>::P *>(b->pos), reinterpret_cast<typename Rows<F32>::V *>(b->rest),
                 b->crushed, b->grain, b->pitch};
-    hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->stream,
-                       b->chunks, b->nChunks, b->cases, S.pos, S.rest, S.crushed, S.pitch,
+    hipLaunchKernelGGL((k_batch_particles<F32, G2P, P2G, RMAP>), dim3(nb), dim3(256), 0, b->lstream,
+                       b->chunks + b->lchunk0, b->lchunks, b->cases, S.pos, S.rest, S.crushed, S.pitch,
                        static_cast<const typename Rows<F32>::G *>(gV), gO, b->gpitch,
                        b->err, b->grain, la, mq, b->storeVB ? 1 : 0);
-    if (b->evAfter) (void)hipEventRecord(b->evAfter, b->stream);
+    if (b->evAfter) (void)hipEventRecord(b->evAfter, b->lstream);
     if (G2P && RMAP == kMapLearned && la.branches != 0) {
         // the deferred points: map, store, and their P2G into the pass's output grid
-        hipLaunchKernelGGL((k_batch_map<F32>), dim3(kMapBlocks), dim3(256), 0, b->stream, S, b->cases, mq.q,
+        hipLaunchKernelGGL((k_batch_map<F32>), dim3(kMapBlocks), dim3(256), 0, b->lstream, S, b->cases, mq.q,
                            mq.count, mq.count == b->mapCount ? b->mapCount + 1 : b->mapCount, gO, b->gpitch,
                            b->err, la);
     }
@@ -1506,9 +1519,13 @@ void launch_prec(icempm_batch *b, unsigned nb, bool g2p, bool p2g, const void *g
 // One particle pass (and, with the learned map, the map kernel after it);
 // parity selects the queue counter the pass appends to.
 // gV: the grid whose rows 1-2 hold the velocities (FP64 storage) or vel32
-hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const void *gV, double *gO, int parity) {
-    const unsigned nb = (unsigned)(((b->nChunks + 7) / 8) * 8);
-    if (b->nChunks <= 0) return hipSuccess;
+hipError_t launch_pass(icempm_batch *b, bool g2p, bool p2g, const void *gV, double *gO, int parity,
+                       hipStream_t st, int chunk0, int chunks) {
+    const unsigned nb = (unsigned)(((chunks + 7) / 8) * 8);
+    if (chunks <= 0) return hipSuccess;
+    b->lstream = st;
+    b->lchunk0 = chunk0;
+    b->lchunks = chunks;
     LearnArgs la{};
     la.wbf = b->mlpW;
     la.pf = b->mlpP;
@@ -1609,9 +1626,18 @@ int upload_chunks(icempm_batch *b) {
     }
     BCHK(b, hipMemcpy(b->chunks, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice), "chunk table");
     b->nChunks = (int)n;
-    // a captured loop holds the table's address and the chunk count (the
-    // grid size of its launches); the table's contents are read at run time
-    if (b->chunks != before || b->nChunks != nBefore) drop_graph(b);
+    // the first chunk of the second half of the cases
+    const int splitBefore = b->grpChunk[1];
+    int split = (int)n;
+    if (b->splitCase > 0)
+        split = (int)(std::lower_bound(b->hccase.begin(), b->hccase.end(), b->splitCase) - b->hccase.begin());
+    b->grpChunk[0] = 0;
+    b->grpChunk[1] = split;
+    b->grpChunk[2] = (int)n;
+    // a captured loop holds the table's address, the chunk count and the
+    // halves' split (the grid sizes of its launches); the table's contents
+    // are read at run time
+    if (b->chunks != before || b->nChunks != nBefore || split != splitBefore) drop_graph(b);
     return ICEMPM_OK;
 }
 
@@ -1738,8 +1764,8 @@ int rebuild_chunks(icempm_batch *b) {
 hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist, int start, int *end) {
     hipError_t e;
     const size_t nblk = (size_t)b->nNodeBlocks;
-    double *A = b->gridA, *B = b->gridB;
-    int *fA = b->blkFlags, *fB = b->blkFlags + nblk, *fV = b->blkFlags + 2 * nblk;
+    double *A0 = b->gridA, *B0 = b->gridB;
+    int *fA0 = b->blkFlags, *fB0 = b->blkFlags + nblk, *fV = b->blkFlags + 2 * nblk;
     if (start < 0 && (e = launch_fill_f64(b->gridA, b->gpitch * 3, 0.0, b->stream)) != hipSuccess) return e;
     if ((e = launch_fill_f64(hist, 2 * (size_t)n * b->hc.size(), 0.0, b->stream)) != hipSuccess) return e;
     if (b->mapCount &&
@@ -1747,47 +1773,72 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
         return e;
     int cur = 0;   // which grid A is
     if (start < 0) {
-        if ((e = launch_pass(b, false, true, b->gridA, b->gridA, 0)) != hipSuccess) return e;
+        if ((e = launch_pass(b, false, true, b->gridA, b->gridA, 0, b->stream, 0, b->nChunks)) != hipSuccess)
+            return e;
         // the grids' and vel32's contents are unknown at a fresh start
         if ((e = launch_fill_i32(b->blkFlags, 3 * nblk, 1, b->stream)) != hipSuccess) return e;
     } else if (start == 1) {
-        std::swap(A, B);
-        std::swap(fA, fB);
+        std::swap(A0, B0);
+        std::swap(fA0, fB0);
         cur = 1;
     }
-    const size_t nc = b->hc.size();
-    for (int i = 0; i < n; ++i) {
-        hipLaunchKernelGGL(k_batch_update, dim3(b->nNodeBlocks), dim3(256), 0, b->stream, b->cases,
-                           b->nodeBlock, A, B, b->gpitch, tTab, i, hist + 2 * nc * (size_t)i, b->vel32, fA, fB,
-                           fV, b->bins);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        const void *gV = b->vel32 ? static_cast<const void *>(b->vel32) : static_cast<const void *>(A);
-        const bool timeIt = b->timingStride > 0 && i % b->timingStride == 0;
-        hipEvent_t t0 = nullptr, t1 = nullptr;
-        if (timeIt) {
-            if (b->tevNext + 2 > b->tev.size()) {
-                for (int q = 0; q < 2; ++q) {
-                    hipEvent_t ev;
-                    if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
-                    b->tev.push_back(ev);
-                }
-            }
-            t0 = b->tev[b->tevNext];
-            t1 = b->tev[b->tevNext + 1];
-            b->tevNext += 2;
-            if ((e = hipEventRecord(t0, b->stream)) != hipSuccess) return e;
-        }
-        b->evAfter = timeIt ? t1 : nullptr;   // recorded right after the particle kernel
-        b->storeVB = (i == n - 1);
-        e = launch_pass(b, true, true, gV, B, i);
-        b->evAfter = nullptr;
-        b->storeVB = false;
-        if (e != hipSuccess) return e;
-        std::swap(A, B);
-        std::swap(fA, fB);
-        cur ^= 1;
+    // The two halves of the cases as two branches, when the cases split,
+    // the map kernel (one queue for all cases) is not in the loop and no
+    // pass is being timed: the second starts after everything above, on
+    // stream2, and the first waits for it at the end.
+    const bool mapKernel = b->rmap == kMapLearned && b->mapBranches != 0;
+    const int groups = (b->splitCase > 0 && !mapKernel && b->timingStride <= 0) ? 2 : 1;
+    if (groups == 2) {
+        if ((e = hipEventRecord(b->evFork, b->stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(b->stream2, b->evFork, 0)) != hipSuccess) return e;
     }
-    *end = cur;
+    const size_t nc = b->hc.size();
+    for (int g = 0; g < groups; ++g) {
+        hipStream_t st = g == 0 ? b->stream : b->stream2;
+        const int c0 = groups == 2 ? b->grpChunk[g] : 0;
+        const int nch = groups == 2 ? b->grpChunk[g + 1] - c0 : b->nChunks;
+        const int k0 = groups == 2 ? b->grpBlock[g] : 0;
+        const int nkb = groups == 2 ? b->grpBlock[g + 1] - k0 : b->nNodeBlocks;
+        double *A = A0, *B = B0;
+        int *fA = fA0, *fB = fB0;
+        for (int i = 0; i < n; ++i) {
+            if (nkb > 0) {
+                hipLaunchKernelGGL(k_batch_update, dim3(nkb), dim3(256), 0, st, b->cases, b->nodeBlock + k0, A, B,
+                                   b->gpitch, tTab, i, hist + 2 * nc * (size_t)i, b->vel32, fA + k0, fB + k0,
+                                   fV + k0, b->bins);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+            }
+            const void *gV = b->vel32 ? static_cast<const void *>(b->vel32) : static_cast<const void *>(A);
+            const bool timeIt = b->timingStride > 0 && i % b->timingStride == 0;
+            hipEvent_t t0 = nullptr, t1 = nullptr;
+            if (timeIt) {
+                if (b->tevNext + 2 > b->tev.size()) {
+                    for (int q = 0; q < 2; ++q) {
+                        hipEvent_t ev;
+                        if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
+                        b->tev.push_back(ev);
+                    }
+                }
+                t0 = b->tev[b->tevNext];
+                t1 = b->tev[b->tevNext + 1];
+                b->tevNext += 2;
+                if ((e = hipEventRecord(t0, st)) != hipSuccess) return e;
+            }
+            b->evAfter = timeIt ? t1 : nullptr;   // recorded right after the particle kernel
+            b->storeVB = (i == n - 1);
+            e = launch_pass(b, true, true, gV, B, i, st, c0, nch);
+            b->evAfter = nullptr;
+            b->storeVB = false;
+            if (e != hipSuccess) return e;
+            std::swap(A, B);
+            std::swap(fA, fB);
+        }
+    }
+    if (groups == 2) {
+        if ((e = hipEventRecord(b->evJoin, b->stream2)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(b->stream, b->evJoin, 0)) != hipSuccess) return e;
+    }
+    *end = cur ^ (n & 1);
     return hipSuccess;
 }
 
@@ -1802,6 +1853,9 @@ int icempm_batch_create(icempm_batch **out, int precision) {
     b->precision = precision;
     hipError_t e = hipSetDevice(0);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evJoin, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->e0);
     if (e == hipSuccess) e = hipEventCreate(&b->e1);
     if (e != hipSuccess) {
@@ -1815,6 +1869,7 @@ int icempm_batch_create(icempm_batch **out, int precision) {
 void icempm_batch_destroy(icempm_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->stream2) (void)hipStreamSynchronize(b->stream2);
     drop_graph(b);
     dfree(b->pos); dfree(b->rest); dfree(b->crushed); dfree(b->gridA); dfree(b->gridB); dfree(b->vel32);
     dfree(b->blkFlags);
@@ -1832,6 +1887,9 @@ void icempm_batch_destroy(icempm_batch *b) {
     for (hipEvent_t ev : b->tev) (void)hipEventDestroy(ev);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
+    if (b->evFork) (void)hipEventDestroy(b->evFork);
+    if (b->evJoin) (void)hipEventDestroy(b->evJoin);
+    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1965,6 +2023,29 @@ int icempm_batch_finalize(icempm_batch *b) {
     b->nPts = (int)pofs;
     b->nNodes = gofs;
     b->nNodeBlocks = (int)nblk.size();
+    // the two halves of the cases: the split nearest to half of the points
+    {
+        const size_t nc = b->hc.size();
+        long acc = 0, best = LONG_MAX;
+        b->splitCase = 0;
+        for (size_t k = 1; k < nc; ++k) {
+            acc += b->hc[k - 1].cp.nPts;
+            const long d = std::labs(2 * acc - pofs);
+            if (d < best) {
+                best = d;
+                b->splitCase = (int)k;
+            }
+        }
+        int kb = (int)nblk.size();
+        for (size_t q = 0; q < nblk.size(); ++q)
+            if (nblk[q].x >= b->splitCase) {
+                kb = (int)q;
+                break;
+            }
+        b->grpBlock[0] = 0;
+        b->grpBlock[1] = kb;
+        b->grpBlock[2] = (int)nblk.size();
+    }
     b->pitch = (pofs + 63) / 64 * 64;
     b->gpitch = ((size_t)gofs + 63) / 64 * 64;
     {
