@@ -57,6 +57,9 @@ struct CaseParams {
     int nbins, binOffset;   // indenter force bins: n_indenter_subdivisions, first double in the bin array
 };
 
+#ifndef ICEMPM_BATCH_CHAINS
+#define ICEMPM_BATCH_CHAINS 2   // independent chains of a batched run (groups of cases)
+#endif
 constexpr int kWX = 48, kWS = 52, kWY = 4;   // window width, padded row stride, rows
 
 // Point rows.  FP64 storage: positions FP64.  FP32 storage: positions as
@@ -1321,16 +1324,20 @@ using namespace icempm::batch;
 struct icempm_batch {
     int precision = ICEMPM_BATCH_F64;
     hipStream_t stream = nullptr;
-    // The cases in two halves of about equal point counts run their substep
-    // loops as two independent branches (the second on stream2), so that one
-    // half's grid update, bound by memory, overlaps the other's particle
-    // pass, bound by issue.  grpChunk / grpBlock: the halves' ranges of the
-    // chunk table and of the node blocks, {0, split, end}.
-    hipStream_t stream2 = nullptr;
-    hipEvent_t evFork = nullptr, evJoin = nullptr;
-    int grpChunk[3] = {0, 0, 0};
-    int grpBlock[3] = {0, 0, 0};
-    int splitCase = 0;   // first case of the second half (0: one branch)
+    // The cases in kChains groups of about equal point counts run their
+    // substep loops as independent chains (group g > 0 on side[g - 1]), so
+    // that one group's grid update, bound by memory, overlaps another's
+    // particle pass, bound by issue.  groupCase: each group's first case;
+    // grpChunk / grpBlock: the groups' ranges of the chunk table and of the
+    // node blocks; chains: the number of groups (1 when there is one case).
+    static constexpr int kChains = ICEMPM_BATCH_CHAINS;
+    hipStream_t side[kChains - 1] = {};
+    hipEvent_t evFork = nullptr;
+    hipEvent_t evJoin[kChains - 1] = {};
+    int chains = 1;
+    int groupCase[kChains + 1] = {};
+    int grpChunk[kChains + 1] = {};
+    int grpBlock[kChains + 1] = {};
     // the launch target of the next particle pass: its stream and chunk range
     hipStream_t lstream = nullptr;
     int lchunk0 = 0, lchunks = 0;
@@ -1626,18 +1633,19 @@ int upload_chunks(icempm_batch *b) {
     }
     BCHK(b, hipMemcpy(b->chunks, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice), "chunk table");
     b->nChunks = (int)n;
-    // the first chunk of the second half of the cases
-    const int splitBefore = b->grpChunk[1];
-    int split = (int)n;
-    if (b->splitCase > 0)
-        split = (int)(std::lower_bound(b->hccase.begin(), b->hccase.end(), b->splitCase) - b->hccase.begin());
-    b->grpChunk[0] = 0;
-    b->grpChunk[1] = split;
-    b->grpChunk[2] = (int)n;
+    // each group's first chunk
+    bool splitMoved = false;
+    for (int g = 0; g <= b->chains; ++g) {
+        const int c = g == b->chains ? (int)n
+                                     : (int)(std::lower_bound(b->hccase.begin(), b->hccase.end(), b->groupCase[g]) -
+                                             b->hccase.begin());
+        splitMoved = splitMoved || c != b->grpChunk[g];
+        b->grpChunk[g] = c;
+    }
     // a captured loop holds the table's address, the chunk count and the
-    // halves' split (the grid sizes of its launches); the table's contents
+    // groups' ranges (the grid sizes of its launches); the table's contents
     // are read at run time
-    if (b->chunks != before || b->nChunks != nBefore || split != splitBefore) drop_graph(b);
+    if (b->chunks != before || b->nChunks != nBefore || splitMoved) drop_graph(b);
     return ICEMPM_OK;
 }
 
@@ -1782,23 +1790,24 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
         std::swap(fA0, fB0);
         cur = 1;
     }
-    // The two halves of the cases as two branches, when the cases split,
+    // The groups of cases as independent chains, when there are several,
     // the map kernel (one queue for all cases) is not in the loop and no
-    // pass is being timed: the second starts after everything above, on
-    // stream2, and the first waits for it at the end.
+    // pass is being timed: the side chains start after everything above,
+    // and the main stream waits for them at the end.
     const bool mapKernel = b->rmap == kMapLearned && b->mapBranches != 0;
-    const int groups = (b->splitCase > 0 && !mapKernel && b->timingStride <= 0) ? 2 : 1;
-    if (groups == 2) {
+    const int groups = (!mapKernel && b->timingStride <= 0) ? b->chains : 1;
+    if (groups > 1) {
         if ((e = hipEventRecord(b->evFork, b->stream)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(b->stream2, b->evFork, 0)) != hipSuccess) return e;
+        for (int g = 1; g < groups; ++g)
+            if ((e = hipStreamWaitEvent(b->side[g - 1], b->evFork, 0)) != hipSuccess) return e;
     }
     const size_t nc = b->hc.size();
     for (int g = 0; g < groups; ++g) {
-        hipStream_t st = g == 0 ? b->stream : b->stream2;
-        const int c0 = groups == 2 ? b->grpChunk[g] : 0;
-        const int nch = groups == 2 ? b->grpChunk[g + 1] - c0 : b->nChunks;
-        const int k0 = groups == 2 ? b->grpBlock[g] : 0;
-        const int nkb = groups == 2 ? b->grpBlock[g + 1] - k0 : b->nNodeBlocks;
+        hipStream_t st = g == 0 ? b->stream : b->side[g - 1];
+        const int c0 = groups > 1 ? b->grpChunk[g] : 0;
+        const int nch = groups > 1 ? b->grpChunk[g + 1] - c0 : b->nChunks;
+        const int k0 = groups > 1 ? b->grpBlock[g] : 0;
+        const int nkb = groups > 1 ? b->grpBlock[g + 1] - k0 : b->nNodeBlocks;
         double *A = A0, *B = B0;
         int *fA = fA0, *fB = fB0;
         for (int i = 0; i < n; ++i) {
@@ -1834,9 +1843,9 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
             std::swap(fA, fB);
         }
     }
-    if (groups == 2) {
-        if ((e = hipEventRecord(b->evJoin, b->stream2)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(b->stream, b->evJoin, 0)) != hipSuccess) return e;
+    for (int g = 1; g < groups; ++g) {
+        if ((e = hipEventRecord(b->evJoin[g - 1], b->side[g - 1])) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(b->stream, b->evJoin[g - 1], 0)) != hipSuccess) return e;
     }
     *end = cur ^ (n & 1);
     return hipSuccess;
@@ -1853,9 +1862,11 @@ int icempm_batch_create(icempm_batch **out, int precision) {
     b->precision = precision;
     hipError_t e = hipSetDevice(0);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evJoin, hipEventDisableTiming);
+    for (int g = 0; g + 1 < icempm_batch::kChains && e == hipSuccess; ++g) {
+        e = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evJoin[g], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = hipEventCreate(&b->e0);
     if (e == hipSuccess) e = hipEventCreate(&b->e1);
     if (e != hipSuccess) {
@@ -1869,7 +1880,8 @@ int icempm_batch_create(icempm_batch **out, int precision) {
 void icempm_batch_destroy(icempm_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+    for (hipStream_t st : b->side)
+        if (st) (void)hipStreamSynchronize(st);
     drop_graph(b);
     dfree(b->pos); dfree(b->rest); dfree(b->crushed); dfree(b->gridA); dfree(b->gridB); dfree(b->vel32);
     dfree(b->blkFlags);
@@ -1888,8 +1900,10 @@ void icempm_batch_destroy(icempm_batch *b) {
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
     if (b->evFork) (void)hipEventDestroy(b->evFork);
-    if (b->evJoin) (void)hipEventDestroy(b->evJoin);
-    if (b->stream2) (void)hipStreamDestroy(b->stream2);
+    for (hipEvent_t ev : b->evJoin)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipStream_t st : b->side)
+        if (st) (void)hipStreamDestroy(st);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -2023,28 +2037,30 @@ int icempm_batch_finalize(icempm_batch *b) {
     b->nPts = (int)pofs;
     b->nNodes = gofs;
     b->nNodeBlocks = (int)nblk.size();
-    // the two halves of the cases: the split nearest to half of the points
+    // the groups of cases: group g starts at the first case whose points
+    // begin at or after g / chains of all points (each group non-empty)
     {
-        const size_t nc = b->hc.size();
-        long acc = 0, best = LONG_MAX;
-        b->splitCase = 0;
-        for (size_t k = 1; k < nc; ++k) {
-            acc += b->hc[k - 1].cp.nPts;
-            const long d = std::labs(2 * acc - pofs);
-            if (d < best) {
-                best = d;
-                b->splitCase = (int)k;
-            }
+        const int nc = (int)b->hc.size();
+        b->chains = std::max(1, std::min(icempm_batch::kChains, nc));
+        b->groupCase[0] = 0;
+        long acc = 0;
+        int k = 0;
+        for (int g = 1; g < b->chains; ++g) {
+            while (k < nc - (b->chains - g) && (acc + b->hc[k].cp.nPts) * b->chains <= (long)g * pofs)
+                acc += b->hc[k++].cp.nPts;
+            k = std::max(k, b->groupCase[g - 1] + 1);
+            b->groupCase[g] = k;
         }
-        int kb = (int)nblk.size();
-        for (size_t q = 0; q < nblk.size(); ++q)
-            if (nblk[q].x >= b->splitCase) {
-                kb = (int)q;
-                break;
-            }
-        b->grpBlock[0] = 0;
-        b->grpBlock[1] = kb;
-        b->grpBlock[2] = (int)nblk.size();
+        b->groupCase[b->chains] = nc;
+        for (int g = 0; g <= b->chains; ++g) {
+            int kb = (int)nblk.size();
+            for (size_t q = 0; q < nblk.size(); ++q)
+                if (nblk[q].x >= b->groupCase[g]) {
+                    kb = (int)q;
+                    break;
+                }
+            b->grpBlock[g] = kb;
+        }
     }
     b->pitch = (pofs + 63) / 64 * 64;
     b->gpitch = ((size_t)gofs + 63) / 64 * 64;
