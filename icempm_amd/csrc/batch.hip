@@ -2037,18 +2037,25 @@ int icempm_batch_finalize(icempm_batch *b) {
     b->nPts = (int)pofs;
     b->nNodes = gofs;
     b->nNodeBlocks = (int)nblk.size();
-    // the groups of cases: group g starts at the first case whose points
-    // begin at or after g / chains of all points (each group non-empty)
+    // the groups of cases: group g starts at the case boundary nearest to
+    // g / chains of all points (each group non-empty)
     {
         const int nc = (int)b->hc.size();
         b->chains = std::max(1, std::min(icempm_batch::kChains, nc));
         b->groupCase[0] = 0;
-        long acc = 0;
+        long acc = 0;   // points of the cases before case k
         int k = 0;
         for (int g = 1; g < b->chains; ++g) {
-            while (k < nc - (b->chains - g) && (acc + b->hc[k].cp.nPts) * b->chains <= (long)g * pofs)
-                acc += b->hc[k++].cp.nPts;
-            k = std::max(k, b->groupCase[g - 1] + 1);
+            const long target = (long)g * pofs;   // in units of 1 / chains points
+            if (k <= b->groupCase[g - 1]) {
+                acc += b->hc[k].cp.nPts;
+                ++k;
+            }
+            while (k < nc - (b->chains - g) &&
+                   std::labs((acc + b->hc[k].cp.nPts) * b->chains - target) <= std::labs(acc * b->chains - target)) {
+                acc += b->hc[k].cp.nPts;
+                ++k;
+            }
             b->groupCase[g] = k;
         }
         b->groupCase[b->chains] = nc;
