@@ -455,6 +455,10 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
     const int4 ch = chunks[c];
     const int beg = ch.x, end = ch.y;
     asm volatile("" ::"s"(ch.z));   // all of the chunk in one load, before the branch
+    // FP32 storage: a wavefront issues its point loads and gathers, and later
+    // its window flush, at raised priority, computes at the base one
+    // (measured: -2.6% per substep; FP64 storage measured slower with it)
+    if constexpr (F32) __builtin_amdgcn_s_setprio(2);
     if (beg + wv * 64 >= end) return;
     const CaseParams cp = cases[ch.z];
     const KParams k = kp_of(cp, err);
@@ -571,6 +575,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
                         }
                     }
             }
+            if constexpr (F32) __builtin_amdgcn_s_setprio(0);
             double v0 = 0, v1 = 0, b00 = 0, b01 = 0, b10 = 0, b11 = 0;
             const double c4 = 4. * cp.h_inv;
             const double wyd0 = wy0 * (0 - fx1), wyd1 = wy1 * (1 - fx1), wyd2 = wy2 * (2 - fx1);
@@ -938,6 +943,7 @@ __global__ __launch_bounds__(256) void k_batch_particles(const int4 *__restrict_
             }
         }
         wsync();
+        if constexpr (F32) __builtin_amdgcn_s_setprio(2);
         if (bcol < bw) {
             for (int ly = brow0; ly < bh; ly += bstep) {
                 const int lx = bcol;
