@@ -27,7 +27,10 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -1598,13 +1601,31 @@ void chunk_table(const icempm_batch *b, const unsigned *cells, std::vector<int> 
     if (nt <= 1 || b->nPts < (1 << 16)) {
         for (size_t ci = 0; ci < nc; ++ci) cut(ci);
     } else {
+        // the calling thread works too; a helper that cannot be started
+        // leaves its share to the others, and an exception (allocation)
+        // in any of them is rethrown here, as the serial loop would throw
         std::atomic<size_t> next{0};
-        std::vector<std::thread> pool;
-        for (size_t t = 0; t < nt; ++t)
-            pool.emplace_back([&] {
+        std::exception_ptr failure;
+        std::mutex failureLock;
+        auto work = [&] {
+            try {
                 for (size_t ci; (ci = next.fetch_add(1)) < nc;) cut(ci);
-            });
+            } catch (...) {
+                std::lock_guard<std::mutex> g(failureLock);
+                if (!failure) failure = std::current_exception();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; ++t) {
+            try {
+                pool.emplace_back(work);
+            } catch (const std::system_error &) {
+                break;
+            }
+        }
+        work();
         for (auto &t : pool) t.join();
+        if (failure) std::rethrow_exception(failure);
     }
     cstart.clear();
     ccase.clear();
