@@ -22,6 +22,7 @@ ICEMPM_E_ARG = 2
 ICEMPM_E_STATE = 3
 ICEMPM_E_NODEV = 4
 ICEMPM_E_IO = 5
+ICEMPM_E_HOST = 6
 MODE_FAST = 0
 MODE_ORDERED = 1
 TUNE_PASS_KIND = 0

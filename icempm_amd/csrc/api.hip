@@ -447,38 +447,40 @@ void readback_done(void *p) {
 extern "C" {
 
 int icempm_create(icempm_ctx **out) {
-    if (!out) return ICEMPM_E_ARG;
-    *out = nullptr;
-    int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess) {
-        g_createError = std::string("initialize() hip error: ") + hipGetErrorString(e);
-        return ICEMPM_E_NODEV;
-    }
-    if (n == 0) {
-        g_createError = "No available HIP devices";
-        return ICEMPM_E_NODEV;
-    }
-    icempm_ctx *c = new icempm_ctx();
-    c->device = 0;
-    e = hipSetDevice(0);
-    if (e == hipSuccess) e = scratch_setup(c->device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&c->evStart);
-    if (e == hipSuccess) e = hipEventCreate(&c->evStop);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evXfer, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMalloc(&c->dErr, sizeof(int));
-    if (e == hipSuccess) e = hipHostMalloc(&c->hostErr, sizeof(int), hipHostMallocDefault);
-    if (e == hipSuccess) e = launch_fill_i32(c->dErr, 1, 0, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-        g_createError = std::string("initialize(): ") + hipGetErrorString(e);
-        icempm_destroy(c);
-        return ICEMPM_E_HIP;
-    }
-    *c->hostErr = 0;
-    *out = c;
-    return ICEMPM_OK;
+    return host_guard(&g_createError, [&]() -> int {
+        if (!out) return ICEMPM_E_ARG;
+        *out = nullptr;
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) {
+            g_createError = std::string("initialize() hip error: ") + hipGetErrorString(e);
+            return ICEMPM_E_NODEV;
+        }
+        if (n == 0) {
+            g_createError = "No available HIP devices";
+            return ICEMPM_E_NODEV;
+        }
+        icempm_ctx *c = new icempm_ctx();
+        c->device = 0;
+        e = hipSetDevice(0);
+        if (e == hipSuccess) e = scratch_setup(c->device);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&c->evStart);
+        if (e == hipSuccess) e = hipEventCreate(&c->evStop);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evXfer, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(&c->dErr, sizeof(int));
+        if (e == hipSuccess) e = hipHostMalloc(&c->hostErr, sizeof(int), hipHostMallocDefault);
+        if (e == hipSuccess) e = launch_fill_i32(c->dErr, 1, 0, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            g_createError = std::string("initialize(): ") + hipGetErrorString(e);
+            icempm_destroy(c);
+            return ICEMPM_E_HIP;
+        }
+        *c->hostErr = 0;
+        *out = c;
+        return ICEMPM_OK;
+    });
 }
 
 void icempm_destroy(icempm_ctx *c) {
@@ -503,412 +505,458 @@ const char *icempm_last_error(const icempm_ctx *c) {
 size_t icempm_simparams_size(void) { return sizeof(icy::SimParams); }
 
 int icempm_test(icempm_ctx *c) {
-    if (!c) return ICEMPM_E_ARG;
-    int *flag = nullptr;
-    HIPCHK(c, hipMalloc(&flag, sizeof(int)), "cuda test");
-    int h = 0;
-    hipError_t e = launch_fill_i32(flag, 1, 0, c->stream);
-    if (e == hipSuccess) e = launch_hello(flag, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(flag);
-    if (e != hipSuccess) return hipfail(c, e, "cuda test");
-    if (h != 1) return fail(c, ICEMPM_E_HIP, "cuda test: kernel did not run");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        int *flag = nullptr;
+        HIPCHK(c, hipMalloc(&flag, sizeof(int)), "cuda test");
+        int h = 0;
+        hipError_t e = launch_fill_i32(flag, 1, 0, c->stream);
+        if (e == hipSuccess) e = launch_hello(flag, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(flag);
+        if (e != hipSuccess) return hipfail(c, e, "cuda test");
+        if (h != 1) return fail(c, ICEMPM_E_HIP, "cuda test: kernel did not run");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_synchronize(icempm_ctx *c) {
-    if (!c) return ICEMPM_E_ARG;
-    HIPCHK(c, hipStreamSynchronize(c->stream), "synchronize");
-    HIPCHK(c, hipDeviceSynchronize(), "synchronize");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        HIPCHK(c, hipStreamSynchronize(c->stream), "synchronize");
+        HIPCHK(c, hipDeviceSynchronize(), "synchronize");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_update_constants(icempm_ctx *c, const void *simparams, size_t nbytes) {
-    if (!c || !simparams) return ICEMPM_E_ARG;
-    if (nbytes != sizeof(icy::SimParams))
-        return fail(c, ICEMPM_E_ARG, "cuda_update_constants: SimParams size mismatch");
-    std::memcpy(&c->prms, simparams, sizeof(icy::SimParams));
-    if (!c->grainTab)
-        HIPCHK(c, hipMalloc(&c->grainTab, (size_t)kGrainEntries * kGrainFields * sizeof(double)),
-               "cuda_update_constants: gprms");
-    {
-        std::vector<double> t;
-        grain_table(c->prms, t);
-        HIPCHK(c, hipMemcpyAsync(c->grainTab, t.data(), t.size() * sizeof(double),
-                                 hipMemcpyHostToDevice, c->stream), "cuda_update_constants: gprms");
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !simparams) return ICEMPM_E_ARG;
+        if (nbytes != sizeof(icy::SimParams))
+            return fail(c, ICEMPM_E_ARG, "cuda_update_constants: SimParams size mismatch");
+        std::memcpy(&c->prms, simparams, sizeof(icy::SimParams));
+        if (!c->grainTab)
+            HIPCHK(c, hipMalloc(&c->grainTab, (size_t)kGrainEntries * kGrainFields * sizeof(double)),
+                   "cuda_update_constants: gprms");
+        {
+            std::vector<double> t;
+            grain_table(c->prms, t);
+            HIPCHK(c, hipMemcpyAsync(c->grainTab, t.data(), t.size() * sizeof(double),
+                                     hipMemcpyHostToDevice, c->stream), "cuda_update_constants: gprms");
+            HIPCHK(c, hipStreamSynchronize(c->stream), "cuda_update_constants: gprms");
+        }
+        c->haveParams = true;
+        c->chunksDirty = true;
+        // gpu_error_indicator <- error_code (gpu_implementation5.cu:474)
+        HIPCHK(c, launch_fill_i32(c->dErr, 1, c->errorCode.load(), c->stream),
+               "gpu_error_indicator initialization");
         HIPCHK(c, hipStreamSynchronize(c->stream), "cuda_update_constants: gprms");
-    }
-    c->haveParams = true;
-    c->chunksDirty = true;
-    // gpu_error_indicator <- error_code (gpu_implementation5.cu:474)
-    HIPCHK(c, launch_fill_i32(c->dErr, 1, c->errorCode.load(), c->stream),
-           "gpu_error_indicator initialization");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "cuda_update_constants: gprms");
-    return ICEMPM_OK;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_allocate_arrays(icempm_ctx *c, size_t nGridNodes, size_t nPoints, int nBins,
                            size_t *nGridPitch, size_t *nPtsPitch, double **host_points,
                            double **host_force) {
-    if (!c || nBins <= 0) return ICEMPM_E_ARG;
-    HIPCHK(c, hipStreamSynchronize(c->stream), "cuda_allocate_arrays");
-    free_all(c);
-    c->gp = round_pitch(nGridNodes);
-    c->pp = round_pitch(nPoints);
-    HIPCHK(c, hipMalloc(&c->gridA, c->gp * kGridRows * sizeof(double)), "cuda_allocate_arrays");
-    HIPCHK(c, hipMalloc(&c->gridB, c->gp * kGridRows * sizeof(double)), "cuda_allocate_arrays");
-    HIPCHK(c, hipMalloc(&c->pts, c->pp * kPtsRows * sizeof(double)), "cuda_allocate_arrays");
-    HIPCHK(c, hipMalloc(&c->bins, (size_t)nBins * sizeof(double)), "cuda_allocate_arrays");
-    HIPCHK(c, hipHostMalloc(&c->hostPts, c->pp * kPtsRows * sizeof(double), hipHostMallocDefault),
-           "cuda_allocate_arrays");
-    HIPCHK(c, hipHostMalloc(&c->hostForce, (size_t)nBins * sizeof(double), hipHostMallocDefault),
-           "cuda_allocate_arrays");
-    std::memset(c->hostPts, 0, c->pp * kPtsRows * sizeof(double));
-    std::memset(c->hostForce, 0, (size_t)nBins * sizeof(double));
-    HIPCHK(c, launch_fill_f64(c->gridA, c->gp * kGridRows, 0.0, c->stream), "cuda_allocate_arrays");
-    HIPCHK(c, launch_fill_f64(c->gridB, c->gp * kGridRows, 0.0, c->stream), "cuda_allocate_arrays");
-    HIPCHK(c, launch_fill_f64(c->pts, c->pp * kPtsRows, 0.0, c->stream), "cuda_allocate_arrays");
-    HIPCHK(c, launch_fill_f64(c->bins, (size_t)nBins, 0.0, c->stream), "cuda_allocate_arrays");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "cuda_allocate_arrays");
-    c->nPtsAlloc = nPoints;
-    c->nGridAlloc = nGridNodes;
-    c->nBinsAlloc = nBins;
-    c->errorCode.store(0);
-    *c->hostErr = 0;
-    HIPCHK(c, launch_fill_i32(c->dErr, 1, 0, c->stream), "cuda_allocate_arrays");
-    if (nGridPitch) *nGridPitch = c->gp;
-    if (nPtsPitch) *nPtsPitch = c->pp;
-    if (host_points) *host_points = c->hostPts;
-    if (host_force) *host_force = c->hostForce;
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || nBins <= 0) return ICEMPM_E_ARG;
+        HIPCHK(c, hipStreamSynchronize(c->stream), "cuda_allocate_arrays");
+        free_all(c);
+        c->gp = round_pitch(nGridNodes);
+        c->pp = round_pitch(nPoints);
+        HIPCHK(c, hipMalloc(&c->gridA, c->gp * kGridRows * sizeof(double)), "cuda_allocate_arrays");
+        HIPCHK(c, hipMalloc(&c->gridB, c->gp * kGridRows * sizeof(double)), "cuda_allocate_arrays");
+        HIPCHK(c, hipMalloc(&c->pts, c->pp * kPtsRows * sizeof(double)), "cuda_allocate_arrays");
+        HIPCHK(c, hipMalloc(&c->bins, (size_t)nBins * sizeof(double)), "cuda_allocate_arrays");
+        HIPCHK(c, hipHostMalloc(&c->hostPts, c->pp * kPtsRows * sizeof(double), hipHostMallocDefault),
+               "cuda_allocate_arrays");
+        HIPCHK(c, hipHostMalloc(&c->hostForce, (size_t)nBins * sizeof(double), hipHostMallocDefault),
+               "cuda_allocate_arrays");
+        std::memset(c->hostPts, 0, c->pp * kPtsRows * sizeof(double));
+        std::memset(c->hostForce, 0, (size_t)nBins * sizeof(double));
+        HIPCHK(c, launch_fill_f64(c->gridA, c->gp * kGridRows, 0.0, c->stream), "cuda_allocate_arrays");
+        HIPCHK(c, launch_fill_f64(c->gridB, c->gp * kGridRows, 0.0, c->stream), "cuda_allocate_arrays");
+        HIPCHK(c, launch_fill_f64(c->pts, c->pp * kPtsRows, 0.0, c->stream), "cuda_allocate_arrays");
+        HIPCHK(c, launch_fill_f64(c->bins, (size_t)nBins, 0.0, c->stream), "cuda_allocate_arrays");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "cuda_allocate_arrays");
+        c->nPtsAlloc = nPoints;
+        c->nGridAlloc = nGridNodes;
+        c->nBinsAlloc = nBins;
+        c->errorCode.store(0);
+        *c->hostErr = 0;
+        HIPCHK(c, launch_fill_i32(c->dErr, 1, 0, c->stream), "cuda_allocate_arrays");
+        if (nGridPitch) *nGridPitch = c->gp;
+        if (nPtsPitch) *nPtsPitch = c->pp;
+        if (host_points) *host_points = c->hostPts;
+        if (host_force) *host_force = c->hostForce;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_transfer_points_to_device(icempm_ctx *c) {
-    if (!c) return ICEMPM_E_ARG;
-    if (!c->pts) return fail(c, ICEMPM_E_STATE, "transfer_points_to_device: arrays not allocated");
-    HIPCHK(c, hipMemcpyAsync(c->pts, c->hostPts, c->pp * kPtsRows * sizeof(double),
-                             hipMemcpyHostToDevice, c->stream), "transfer_points_to_device");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "transfer_points_to_device");
-    std::memset(c->hostForce, 0, (size_t)c->nBinsAlloc * sizeof(double));
-    c->chunksDirty = true;
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        if (!c->pts) return fail(c, ICEMPM_E_STATE, "transfer_points_to_device: arrays not allocated");
+        HIPCHK(c, hipMemcpyAsync(c->pts, c->hostPts, c->pp * kPtsRows * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream), "transfer_points_to_device");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "transfer_points_to_device");
+        std::memset(c->hostForce, 0, (size_t)c->nBinsAlloc * sizeof(double));
+        c->chunksDirty = true;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_reset_grid(icempm_ctx *c) {
-    if (!c) return ICEMPM_E_ARG;
-    if (!c->gridA) return fail(c, ICEMPM_E_STATE, "cuda_reset_grid error: arrays not allocated");
-    HIPCHK(c, launch_fill_f64(c->gridA, c->gp * kGridRows, 0.0, c->stream), "cuda_reset_grid error");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        if (!c->gridA) return fail(c, ICEMPM_E_STATE, "cuda_reset_grid error: arrays not allocated");
+        HIPCHK(c, launch_fill_f64(c->gridA, c->gp * kGridRows, 0.0, c->stream), "cuda_reset_grid error");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_reset_indenter_force_accumulator(icempm_ctx *c) {
-    if (!c) return ICEMPM_E_ARG;
-    if (!c->bins) return fail(c, ICEMPM_E_STATE, "cuda_reset_grid error: arrays not allocated");
-    HIPCHK(c, launch_fill_f64(c->bins, (size_t)c->nBinsAlloc, 0.0, c->stream),
-           "cuda_reset_grid error");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        if (!c->bins) return fail(c, ICEMPM_E_STATE, "cuda_reset_grid error: arrays not allocated");
+        HIPCHK(c, launch_fill_f64(c->bins, (size_t)c->nBinsAlloc, 0.0, c->stream),
+               "cuda_reset_grid error");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_p2g(icempm_ctx *c) {
-    int rc = check_ready(c, "cuda_p2g");
-    if (rc) return rc;
-    const KParams k = make_kparams(c);
-    if (c->mode == ICEMPM_MODE_ORDERED) {
-        if ((rc = ensure_ordered(c))) return rc;
-        HIPCHK(c, ordered_p2g(k, c->gridA, c->ord, c->stream), "cuda_p2g");
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        int rc = check_ready(c, "cuda_p2g");
+        if (rc) return rc;
+        const KParams k = make_kparams(c);
+        if (c->mode == ICEMPM_MODE_ORDERED) {
+            if ((rc = ensure_ordered(c))) return rc;
+            HIPCHK(c, ordered_p2g(k, c->gridA, c->ord, c->stream), "cuda_p2g");
+            return ICEMPM_OK;
+        }
+        if ((rc = ensure_chunks(c))) return rc;
+        HIPCHK(c, particles(c, k, false, true, c->gridA, c->gridA, 0), "cuda_p2g");
         return ICEMPM_OK;
-    }
-    if ((rc = ensure_chunks(c))) return rc;
-    HIPCHK(c, particles(c, k, false, true, c->gridA, c->gridA, 0), "cuda_p2g");
-    return ICEMPM_OK;
+    });
 }
 
 int icempm_update_nodes(icempm_ctx *c, double indenter_x, double indenter_y) {
-    int rc = check_ready(c, "cuda_update_nodes");
-    if (rc) return rc;
-    const KParams k = make_kparams(c);
-    if (c->mode == ICEMPM_MODE_ORDERED) {
-        if ((rc = ensure_ordered(c))) return rc;
-        HIPCHK(c, launch_update(k, c->gridA, indenter_x, indenter_y, nullptr, nullptr, true,
-                                c->ord.nodeBin, c->ord.nodeF, c->stream), "cuda_update_nodes");
-        HIPCHK(c, ordered_fold_forces(k, c->ord, nullptr, c->stream), "cuda_update_nodes");
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        int rc = check_ready(c, "cuda_update_nodes");
+        if (rc) return rc;
+        const KParams k = make_kparams(c);
+        if (c->mode == ICEMPM_MODE_ORDERED) {
+            if ((rc = ensure_ordered(c))) return rc;
+            HIPCHK(c, launch_update(k, c->gridA, indenter_x, indenter_y, nullptr, nullptr, true,
+                                    c->ord.nodeBin, c->ord.nodeF, c->stream), "cuda_update_nodes");
+            HIPCHK(c, ordered_fold_forces(k, c->ord, nullptr, c->stream), "cuda_update_nodes");
+            return ICEMPM_OK;
+        }
+        HIPCHK(c, launch_update(k, c->gridA, indenter_x, indenter_y, nullptr, nullptr, false, nullptr,
+                                nullptr, c->stream), "cuda_update_nodes");
         return ICEMPM_OK;
-    }
-    HIPCHK(c, launch_update(k, c->gridA, indenter_x, indenter_y, nullptr, nullptr, false, nullptr,
-                            nullptr, c->stream), "cuda_update_nodes");
-    return ICEMPM_OK;
+    });
 }
 
 int icempm_g2p(icempm_ctx *c, int recordPQ) {
-    int rc = check_ready(c, "cuda_g2p");
-    if (rc) return rc;
-    if ((rc = ensure_chunks(c))) return rc;
-    const KParams k = make_kparams(c);
-    HIPCHK(c, particles(c, k, true, false, c->gridA, nullptr, recordPQ ? 1 : 0), "cuda_g2p");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        int rc = check_ready(c, "cuda_g2p");
+        if (rc) return rc;
+        if ((rc = ensure_chunks(c))) return rc;
+        const KParams k = make_kparams(c);
+        HIPCHK(c, particles(c, k, true, false, c->gridA, nullptr, recordPQ ? 1 : 0), "cuda_g2p");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_run_substeps(icempm_ctx *c, int first_step, int n, double *t,
                         double indenter_x_initial, double indenter_y, int reset_bins_at_cycle) {
-    int rc = check_ready(c, "run_substeps");
-    if (rc) return rc;
-    if (!t || n < 0 || first_step < 0) return fail(c, ICEMPM_E_ARG, "run_substeps: bad arguments");
-    const int N = c->prms.UpdateEveryNthStep;
-    if (N <= 0) return fail(c, ICEMPM_E_ARG, "run_substeps: UpdateEveryNthStep must be positive");
-    const size_t histNeed = 2 * (size_t)n;
-    if (histNeed > c->histCap) {
-        dfree(c->hist);
-        const size_t cap = histNeed < 64 ? 64 : histNeed;
-        HIPCHK(c, hipMalloc(&c->hist, cap * sizeof(double)), "run_substeps");
-        c->histCap = cap;
-    }
-    HIPCHK(c, launch_fill_f64(c->hist, histNeed, 0.0, c->stream), "run_substeps");
-    c->lastRunSubsteps = n;
-    if (n == 0) return ICEMPM_OK;
-    const double dt = c->prms.InitialTimeStep, V = c->prms.IndVelocity;
-    double tt = *t;
-    KParams k = make_kparams(c);
-    const bool ordered = c->mode == ICEMPM_MODE_ORDERED;
-    if (ordered) {
-        if ((rc = ensure_ordered(c))) return rc;
-    }
-    if ((rc = ensure_chunks(c))) return rc;
-    if (!ordered) {
-        HIPCHK(c, launch_fill_f64(c->gridA, c->gp * kGridRows, 0.0, c->stream), "run_substeps");
-        HIPCHK(c, particles(c, k, false, true, c->gridA, c->gridA, 0), "run_substeps");
-    }
-    for (int i = 0; i < n; ++i) {
-        const int s = first_step + i;
-        if (reset_bins_at_cycle && s % N == 0)
-            HIPCHK(c, launch_fill_f64(c->bins, (size_t)c->nBinsAlloc, 0.0, c->stream), "run_substeps");
-        tt += dt;
-        const double ix = indenter_x_initial + tt * V;
-        const int rec = ((s + 1) % N == 0) ? 1 : 0;
-        double *h2 = c->hist + 2 * (size_t)i;
-        const int timeIt = c->timing && (i % c->timingStride == 0);
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        int rc = check_ready(c, "run_substeps");
+        if (rc) return rc;
+        if (!t || n < 0 || first_step < 0) return fail(c, ICEMPM_E_ARG, "run_substeps: bad arguments");
+        const int N = c->prms.UpdateEveryNthStep;
+        if (N <= 0) return fail(c, ICEMPM_E_ARG, "run_substeps: UpdateEveryNthStep must be positive");
+        const size_t histNeed = 2 * (size_t)n;
+        if (histNeed > c->histCap) {
+            dfree(c->hist);
+            const size_t cap = histNeed < 64 ? 64 : histNeed;
+            HIPCHK(c, hipMalloc(&c->hist, cap * sizeof(double)), "run_substeps");
+            c->histCap = cap;
+        }
+        HIPCHK(c, launch_fill_f64(c->hist, histNeed, 0.0, c->stream), "run_substeps");
+        c->lastRunSubsteps = n;
+        if (n == 0) return ICEMPM_OK;
+        const double dt = c->prms.InitialTimeStep, V = c->prms.IndVelocity;
+        double tt = *t;
+        KParams k = make_kparams(c);
+        const bool ordered = c->mode == ICEMPM_MODE_ORDERED;
         if (ordered) {
+            if ((rc = ensure_ordered(c))) return rc;
+        }
+        if ((rc = ensure_chunks(c))) return rc;
+        if (!ordered) {
             HIPCHK(c, launch_fill_f64(c->gridA, c->gp * kGridRows, 0.0, c->stream), "run_substeps");
-            k.pts = c->pts;
-            HIPCHK(c, ordered_p2g(k, c->gridA, c->ord, c->stream), "run_substeps");
-            HIPCHK(c, launch_update(k, c->gridA, ix, indenter_y, nullptr, nullptr, true,
-                                    c->ord.nodeBin, c->ord.nodeF, c->stream), "run_substeps");
-            HIPCHK(c, ordered_fold_forces(k, c->ord, h2, c->stream), "run_substeps");
-            if ((rc = launch_particles_timed(c, k, true, false, c->gridA, nullptr, rec, timeIt)))
-                return rc;
-        } else {
-            const bool last = (i == n - 1);
-            HIPCHK(c, launch_update(k, c->gridA, ix, indenter_y, h2, last ? nullptr : c->gridB,
-                                    false, nullptr, nullptr, c->stream), "run_substeps");
-            if (!last) {
-                if ((rc = launch_particles_timed(c, k, true, true, c->gridA, c->gridB, rec, timeIt)))
-                    return rc;
-                std::swap(c->gridA, c->gridB);
-            } else {
+            HIPCHK(c, particles(c, k, false, true, c->gridA, c->gridA, 0), "run_substeps");
+        }
+        for (int i = 0; i < n; ++i) {
+            const int s = first_step + i;
+            if (reset_bins_at_cycle && s % N == 0)
+                HIPCHK(c, launch_fill_f64(c->bins, (size_t)c->nBinsAlloc, 0.0, c->stream), "run_substeps");
+            tt += dt;
+            const double ix = indenter_x_initial + tt * V;
+            const int rec = ((s + 1) % N == 0) ? 1 : 0;
+            double *h2 = c->hist + 2 * (size_t)i;
+            const int timeIt = c->timing && (i % c->timingStride == 0);
+            if (ordered) {
+                HIPCHK(c, launch_fill_f64(c->gridA, c->gp * kGridRows, 0.0, c->stream), "run_substeps");
+                k.pts = c->pts;
+                HIPCHK(c, ordered_p2g(k, c->gridA, c->ord, c->stream), "run_substeps");
+                HIPCHK(c, launch_update(k, c->gridA, ix, indenter_y, nullptr, nullptr, true,
+                                        c->ord.nodeBin, c->ord.nodeF, c->stream), "run_substeps");
+                HIPCHK(c, ordered_fold_forces(k, c->ord, h2, c->stream), "run_substeps");
                 if ((rc = launch_particles_timed(c, k, true, false, c->gridA, nullptr, rec, timeIt)))
                     return rc;
+            } else {
+                const bool last = (i == n - 1);
+                HIPCHK(c, launch_update(k, c->gridA, ix, indenter_y, h2, last ? nullptr : c->gridB,
+                                        false, nullptr, nullptr, c->stream), "run_substeps");
+                if (!last) {
+                    if ((rc = launch_particles_timed(c, k, true, true, c->gridA, c->gridB, rec, timeIt)))
+                        return rc;
+                    std::swap(c->gridA, c->gridB);
+                } else {
+                    if ((rc = launch_particles_timed(c, k, true, false, c->gridA, nullptr, rec, timeIt)))
+                        return rc;
+                }
             }
         }
-    }
-    *t = tt;
-    if (c->timing) {
-        if ((rc = collect_timing(c))) return rc;
-    }
-    return ICEMPM_OK;
+        *t = tt;
+        if (c->timing) {
+            if ((rc = collect_timing(c))) return rc;
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_force_history(icempm_ctx *c, double *out, int n) {
-    if (!c || !out || n < 0) return ICEMPM_E_ARG;
-    if (n > c->lastRunSubsteps) return fail(c, ICEMPM_E_ARG, "force_history: n exceeds the last run");
-    if (n == 0) return ICEMPM_OK;
-    HIPCHK(c, hipMemcpyAsync(out, c->hist, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
-                             c->stream), "force_history");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "force_history");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !out || n < 0) return ICEMPM_E_ARG;
+        if (n > c->lastRunSubsteps) return fail(c, ICEMPM_E_ARG, "force_history: n exceeds the last run");
+        if (n == 0) return ICEMPM_OK;
+        HIPCHK(c, hipMemcpyAsync(out, c->hist, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream), "force_history");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "force_history");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_transfer_from_device(icempm_ctx *c, icempm_callback cb, void *user) {
-    if (!c) return ICEMPM_E_ARG;
-    if (!c->pts) return fail(c, ICEMPM_E_STATE, "cuda_transfer_from_device: arrays not allocated");
-    HIPCHK(c, hipMemcpyAsync(c->hostPts, c->pts, c->pp * kPtsRows * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream), "cuda_transfer_from_device");
-    HIPCHK(c, hipMemcpyAsync(c->hostForce, c->bins, (size_t)c->nBinsAlloc * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream), "cuda_transfer_from_device");
-    HIPCHK(c, hipMemcpyAsync(c->hostErr, c->dErr, sizeof(int), hipMemcpyDeviceToHost, c->stream),
-           "cuda_transfer_from_device");
-    HIPCHK(c, hipEventRecord(c->evXfer, c->stream), "cuda_transfer_from_device");
-    c->xferPending = true;
-    ReadbackJob *j = new ReadbackJob{c, cb, user};
-    hipError_t e = hipLaunchHostFunc(c->stream, readback_done, j);
-    if (e != hipSuccess) {
-        delete j;
-        return hipfail(c, e, "cuda_transfer_from_device");
-    }
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        if (!c->pts) return fail(c, ICEMPM_E_STATE, "cuda_transfer_from_device: arrays not allocated");
+        HIPCHK(c, hipMemcpyAsync(c->hostPts, c->pts, c->pp * kPtsRows * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream), "cuda_transfer_from_device");
+        HIPCHK(c, hipMemcpyAsync(c->hostForce, c->bins, (size_t)c->nBinsAlloc * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream), "cuda_transfer_from_device");
+        HIPCHK(c, hipMemcpyAsync(c->hostErr, c->dErr, sizeof(int), hipMemcpyDeviceToHost, c->stream),
+               "cuda_transfer_from_device");
+        HIPCHK(c, hipEventRecord(c->evXfer, c->stream), "cuda_transfer_from_device");
+        c->xferPending = true;
+        ReadbackJob *j = new ReadbackJob{c, cb, user};
+        hipError_t e = hipLaunchHostFunc(c->stream, readback_done, j);
+        if (e != hipSuccess) {
+            delete j;
+            return hipfail(c, e, "cuda_transfer_from_device");
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_transfer_wait(icempm_ctx *c) {
-    if (!c) return ICEMPM_E_ARG;
-    if (!c->xferPending) return ICEMPM_OK;
-    // the copies only: the completion callback queued after them may still run
-    HIPCHK(c, hipEventSynchronize(c->evXfer), "cuda_transfer_from_device");
-    c->xferPending = false;
-    c->errorCode.store(*c->hostErr);
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        if (!c->xferPending) return ICEMPM_OK;
+        // the copies only: the completion callback queued after them may still run
+        HIPCHK(c, hipEventSynchronize(c->evXfer), "cuda_transfer_from_device");
+        c->xferPending = false;
+        c->errorCode.store(*c->hostErr);
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_error_code(const icempm_ctx *c) { return c ? c->errorCode.load() : 0; }
 
 int icempm_read_error_flag(icempm_ctx *c, int *flag) {
-    if (!c || !flag) return ICEMPM_E_ARG;
-    HIPCHK(c, hipMemcpyAsync(c->hostErr, c->dErr, sizeof(int), hipMemcpyDeviceToHost, c->stream),
-           "read_error_flag");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "read_error_flag");
-    *flag = *c->hostErr;
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !flag) return ICEMPM_E_ARG;
+        HIPCHK(c, hipMemcpyAsync(c->hostErr, c->dErr, sizeof(int), hipMemcpyDeviceToHost, c->stream),
+               "read_error_flag");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "read_error_flag");
+        *flag = *c->hostErr;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_event_record_start(icempm_ctx *c) {
-    if (!c) return ICEMPM_E_ARG;
-    HIPCHK(c, hipEventRecord(c->evStart, c->stream), "event record");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        HIPCHK(c, hipEventRecord(c->evStart, c->stream), "event record");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_event_record_stop(icempm_ctx *c) {
-    if (!c) return ICEMPM_E_ARG;
-    HIPCHK(c, hipEventRecord(c->evStop, c->stream), "event record");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        HIPCHK(c, hipEventRecord(c->evStop, c->stream), "event record");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_event_elapsed_ms(icempm_ctx *c, float *ms) {
-    if (!c || !ms) return ICEMPM_E_ARG;
-    HIPCHK(c, hipEventSynchronize(c->evStop), "event elapsed");
-    HIPCHK(c, hipEventElapsedTime(ms, c->evStart, c->evStop), "event elapsed");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !ms) return ICEMPM_E_ARG;
+        HIPCHK(c, hipEventSynchronize(c->evStop), "event elapsed");
+        HIPCHK(c, hipEventElapsedTime(ms, c->evStart, c->evStop), "event elapsed");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_sort_points(icempm_ctx *c, int *perm_out, int *keys_out) {
-    int rc = check_ready(c, "sort_points");
-    if (rc) return rc;
-    const int n = c->prms.nPts;
-    if ((rc = ensure_sort_scratch(c, (size_t)n))) return rc;
-    if (!c->ptsTmp)
-        HIPCHK(c, hipMalloc(&c->ptsTmp, c->pp * kPtsRows * sizeof(double)), "sort_points");
-    HIPCHK(c, launch_point_keys(c->pts, c->pp, n, c->prms.cellsize_inv, c->prms.GridX, c->keys,
-                                c->vals, c->stream), "sort_points");
-    HIPCHK(c, radix_sort_pairs(c->keys, c->vals, c->keys2, c->vals2, (size_t)n, c->sortTmp,
-                               c->stream), "sort_points");
-    HIPCHK(c, launch_permute_points(c->pts, c->ptsTmp, c->pp, n, c->vals, c->stream), "sort_points");
-    std::swap(c->pts, c->ptsTmp);
-    if (perm_out && n > 0)
-        HIPCHK(c, hipMemcpyAsync(perm_out, c->vals, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
-                                 c->stream), "sort_points");
-    if (keys_out && n > 0)
-        HIPCHK(c, hipMemcpyAsync(keys_out, c->keys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
-                                 c->stream), "sort_points");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "sort_points");
-    c->chunksDirty = true;
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        int rc = check_ready(c, "sort_points");
+        if (rc) return rc;
+        const int n = c->prms.nPts;
+        if ((rc = ensure_sort_scratch(c, (size_t)n))) return rc;
+        if (!c->ptsTmp)
+            HIPCHK(c, hipMalloc(&c->ptsTmp, c->pp * kPtsRows * sizeof(double)), "sort_points");
+        HIPCHK(c, launch_point_keys(c->pts, c->pp, n, c->prms.cellsize_inv, c->prms.GridX, c->keys,
+                                    c->vals, c->stream), "sort_points");
+        HIPCHK(c, radix_sort_pairs(c->keys, c->vals, c->keys2, c->vals2, (size_t)n, c->sortTmp,
+                                   c->stream), "sort_points");
+        HIPCHK(c, launch_permute_points(c->pts, c->ptsTmp, c->pp, n, c->vals, c->stream), "sort_points");
+        std::swap(c->pts, c->ptsTmp);
+        if (perm_out && n > 0)
+            HIPCHK(c, hipMemcpyAsync(perm_out, c->vals, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
+                                     c->stream), "sort_points");
+        if (keys_out && n > 0)
+            HIPCHK(c, hipMemcpyAsync(keys_out, c->keys, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
+                                     c->stream), "sort_points");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "sort_points");
+        c->chunksDirty = true;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_sort_raw_points(icempm_ctx *c, float *xy, int16_t *grains, int n, double hinv,
                            int gridX) {
-    if (!c || n < 0 || (n > 0 && (!xy || !grains))) return ICEMPM_E_ARG;
-    if (n <= 1) return ICEMPM_OK;
-    int rc = ensure_sort_scratch(c, (size_t)n);
-    if (rc) return rc;
-    float *dxy = nullptr, *dxy2 = nullptr;
-    int16_t *dg = nullptr, *dg2 = nullptr;
-    hipError_t e = hipMalloc(&dxy, 2 * (size_t)n * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&dxy2, 2 * (size_t)n * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&dg, (size_t)n * sizeof(int16_t));
-    if (e == hipSuccess) e = hipMalloc(&dg2, (size_t)n * sizeof(int16_t));
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(dxy, xy, 2 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(dg, grains, (size_t)n * sizeof(int16_t), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = launch_raw_keys(dxy, n, hinv, gridX, c->keys, c->vals, c->stream);
-    if (e == hipSuccess)
-        e = radix_sort_pairs(c->keys, c->vals, c->keys2, c->vals2, (size_t)n, c->sortTmp, c->stream);
-    if (e == hipSuccess) e = launch_permute_raw(dxy, dg, dxy2, dg2, n, c->vals, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(xy, dxy2, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(grains, dg2, (size_t)n * sizeof(int16_t), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipStreamSynchronize(c->stream);
-    dfree(dxy); dfree(dxy2); dfree(dg); dfree(dg2);
-    if (e != hipSuccess) return hipfail(c, e, "sort_raw_points");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || n < 0 || (n > 0 && (!xy || !grains))) return ICEMPM_E_ARG;
+        if (n <= 1) return ICEMPM_OK;
+        int rc = ensure_sort_scratch(c, (size_t)n);
+        if (rc) return rc;
+        float *dxy = nullptr, *dxy2 = nullptr;
+        int16_t *dg = nullptr, *dg2 = nullptr;
+        hipError_t e = hipMalloc(&dxy, 2 * (size_t)n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&dxy2, 2 * (size_t)n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&dg, (size_t)n * sizeof(int16_t));
+        if (e == hipSuccess) e = hipMalloc(&dg2, (size_t)n * sizeof(int16_t));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dxy, xy, 2 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dg, grains, (size_t)n * sizeof(int16_t), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = launch_raw_keys(dxy, n, hinv, gridX, c->keys, c->vals, c->stream);
+        if (e == hipSuccess)
+            e = radix_sort_pairs(c->keys, c->vals, c->keys2, c->vals2, (size_t)n, c->sortTmp, c->stream);
+        if (e == hipSuccess) e = launch_permute_raw(dxy, dg, dxy2, dg2, n, c->vals, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(xy, dxy2, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(grains, dg2, (size_t)n * sizeof(int16_t), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        dfree(dxy); dfree(dxy2); dfree(dg); dfree(dg2);
+        if (e != hipSuccess) return hipfail(c, e, "sort_raw_points");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_sort_raw_points_noalloc(icempm_ctx *c, float *xy, int16_t *grains, int n, double hinv,
                                    int gridX) {
-    if (!c || n < 0 || (n > 0 && (!xy || !grains))) return ICEMPM_E_ARG;
-    if ((size_t)n > kNoAllocMaxPoints)
-        return fail(c, ICEMPM_E_ARG, "sort_raw_points_noalloc: n = " + std::to_string(n) +
-                                         " exceeds ICEMPM_NOALLOC_MAX_POINTS = " + std::to_string(kNoAllocMaxPoints));
-    if (n <= 1) return ICEMPM_OK;
-    if (radix_tmp_ints((size_t)n) > kRadixTmpInts)
-        return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: radix scratch bound too small");
-    unsigned char *base = scratch_of(c->device);
-    if (!base) return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: scratch not set up on this device");
-    unsigned char *q = base;
-    auto take = [&q](size_t bytes) {
-        unsigned char *r = q;
-        q += al256(bytes);
-        return r;
-    };
-    float *dxy = reinterpret_cast<float *>(take(kNoAllocMaxPoints * 8));
-    float *dxy2 = reinterpret_cast<float *>(take(kNoAllocMaxPoints * 8));
-    int16_t *dg = reinterpret_cast<int16_t *>(take(kNoAllocMaxPoints * 2));
-    int16_t *dg2 = reinterpret_cast<int16_t *>(take(kNoAllocMaxPoints * 2));
-    int *keys = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
-    int *vals = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
-    int *keys2 = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
-    int *vals2 = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
-    int *tmp = reinterpret_cast<int *>(take(kRadixTmpInts * 4));
-    if ((size_t)(q - base) > kScratchBytes)
-        return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: scratch layout exceeds the array");
-    // every piece is written before it is read (keys and values by k_raw_keys,
-    // histograms and scan sums by their kernels each pass): nothing to clear
-    hipError_t e = hipMemcpyAsync(dxy, xy, 2 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(dg, grains, (size_t)n * sizeof(int16_t), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = launch_raw_keys(dxy, n, hinv, gridX, keys, vals, c->stream);
-    if (e == hipSuccess) e = radix_sort_pairs(keys, vals, keys2, vals2, (size_t)n, tmp, c->stream);
-    if (e == hipSuccess) e = launch_permute_raw(dxy, dg, dxy2, dg2, n, vals, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(xy, dxy2, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(grains, dg2, (size_t)n * sizeof(int16_t), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-        (void)hipStreamSynchronize(c->stream);
-        return hipfail(c, e, "sort_raw_points_noalloc");
-    }
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || n < 0 || (n > 0 && (!xy || !grains))) return ICEMPM_E_ARG;
+        if ((size_t)n > kNoAllocMaxPoints)
+            return fail(c, ICEMPM_E_ARG, "sort_raw_points_noalloc: n = " + std::to_string(n) +
+                                             " exceeds ICEMPM_NOALLOC_MAX_POINTS = " + std::to_string(kNoAllocMaxPoints));
+        if (n <= 1) return ICEMPM_OK;
+        if (radix_tmp_ints((size_t)n) > kRadixTmpInts)
+            return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: radix scratch bound too small");
+        unsigned char *base = scratch_of(c->device);
+        if (!base) return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: scratch not set up on this device");
+        unsigned char *q = base;
+        auto take = [&q](size_t bytes) {
+            unsigned char *r = q;
+            q += al256(bytes);
+            return r;
+        };
+        float *dxy = reinterpret_cast<float *>(take(kNoAllocMaxPoints * 8));
+        float *dxy2 = reinterpret_cast<float *>(take(kNoAllocMaxPoints * 8));
+        int16_t *dg = reinterpret_cast<int16_t *>(take(kNoAllocMaxPoints * 2));
+        int16_t *dg2 = reinterpret_cast<int16_t *>(take(kNoAllocMaxPoints * 2));
+        int *keys = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
+        int *vals = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
+        int *keys2 = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
+        int *vals2 = reinterpret_cast<int *>(take(kNoAllocMaxPoints * 4));
+        int *tmp = reinterpret_cast<int *>(take(kRadixTmpInts * 4));
+        if ((size_t)(q - base) > kScratchBytes)
+            return fail(c, ICEMPM_E_STATE, "sort_raw_points_noalloc: scratch layout exceeds the array");
+        // every piece is written before it is read (keys and values by k_raw_keys,
+        // histograms and scan sums by their kernels each pass): nothing to clear
+        hipError_t e = hipMemcpyAsync(dxy, xy, 2 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dg, grains, (size_t)n * sizeof(int16_t), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = launch_raw_keys(dxy, n, hinv, gridX, keys, vals, c->stream);
+        if (e == hipSuccess) e = radix_sort_pairs(keys, vals, keys2, vals2, (size_t)n, tmp, c->stream);
+        if (e == hipSuccess) e = launch_permute_raw(dxy, dg, dxy2, dg2, n, vals, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(xy, dxy2, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(grains, dg2, (size_t)n * sizeof(int16_t), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(c->stream);
+            return hipfail(c, e, "sort_raw_points_noalloc");
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_set_mode(icempm_ctx *c, int mode) {
-    if (!c || (mode != ICEMPM_MODE_FAST && mode != ICEMPM_MODE_ORDERED)) return ICEMPM_E_ARG;
-    c->mode = mode;
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || (mode != ICEMPM_MODE_FAST && mode != ICEMPM_MODE_ORDERED)) return ICEMPM_E_ARG;
+        c->mode = mode;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_get_mode(const icempm_ctx *c) { return c ? c->mode : -1; }
 
 int icempm_set_tuning(icempm_ctx *c, int knob, int value) {
-    if (!c) return ICEMPM_E_ARG;
-    switch (knob) {
-    case ICEMPM_TUNE_PASS_KIND:
-        if (value < 0 || value > 2) return fail(c, ICEMPM_E_ARG, "set_tuning: pass kind is 0, 1 or 2");
-        if ((value == 0) != (c->passKind == 0)) c->chunksDirty = true;   // the chunk rule differs
-        c->passKind = value;
-        return ICEMPM_OK;
-    default:
-        return fail(c, ICEMPM_E_ARG, "set_tuning: unknown knob");
-    }
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c) return ICEMPM_E_ARG;
+        switch (knob) {
+        case ICEMPM_TUNE_PASS_KIND:
+            if (value < 0 || value > 2) return fail(c, ICEMPM_E_ARG, "set_tuning: pass kind is 0, 1 or 2");
+            if ((value == 0) != (c->passKind == 0)) c->chunksDirty = true;   // the chunk rule differs
+            c->passKind = value;
+            return ICEMPM_OK;
+        default:
+            return fail(c, ICEMPM_E_ARG, "set_tuning: unknown knob");
+        }
+    });
 }
 
 int icempm_get_tuning(const icempm_ctx *c, int knob) {
@@ -920,83 +968,97 @@ int icempm_get_tuning(const icempm_ctx *c, int knob) {
 }
 
 int icempm_kernel_timing(icempm_ctx *c, int enable) {
-    if (!c || enable < 0) return ICEMPM_E_ARG;
-    c->timing = enable > 0;
-    c->timingStride = enable > 0 ? enable : 1;
-    c->timedMs = 0;
-    c->timedLaunches = 0;
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || enable < 0) return ICEMPM_E_ARG;
+        c->timing = enable > 0;
+        c->timingStride = enable > 0 ? enable : 1;
+        c->timedMs = 0;
+        c->timedLaunches = 0;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_kernel_timing_read(icempm_ctx *c, double *total_ms, int *launches) {
-    if (!c || !total_ms || !launches) return ICEMPM_E_ARG;
-    *total_ms = c->timedMs;
-    *launches = c->timedLaunches;
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !total_ms || !launches) return ICEMPM_E_ARG;
+        *total_ms = c->timedMs;
+        *launches = c->timedLaunches;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_measure_copy(icempm_ctx *c, size_t nbytes, int reps, double *gbps) {
-    if (!c || !gbps || reps <= 0 || nbytes < 1024) return ICEMPM_E_ARG;
-    const size_t n = nbytes / sizeof(double) / 2 * 2;
-    double *a = nullptr, *b = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipMalloc(&a, n * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&b, n * sizeof(double));
-    if (e == hipSuccess) e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = launch_fill_f64(a, n, 1.0, c->stream);
-    if (e == hipSuccess) e = launch_copy(a, b, n, c->stream);   // warm-up
-    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(a, b, n, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-    float ms = 0;
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    (void)hipStreamSynchronize(c->stream);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    dfree(a);
-    dfree(b);
-    if (e != hipSuccess) return hipfail(c, e, "measure_copy");
-    // bytes read + bytes written per copy
-    *gbps = 2.0 * (double)n * sizeof(double) * reps / (ms * 1e-3) / 1e9;
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !gbps || reps <= 0 || nbytes < 1024) return ICEMPM_E_ARG;
+        const size_t n = nbytes / sizeof(double) / 2 * 2;
+        double *a = nullptr, *b = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipMalloc(&a, n * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&b, n * sizeof(double));
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess) e = launch_fill_f64(a, n, 1.0, c->stream);
+        if (e == hipSuccess) e = launch_copy(a, b, n, c->stream);   // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(a, b, n, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        (void)hipStreamSynchronize(c->stream);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        dfree(a);
+        dfree(b);
+        if (e != hipSuccess) return hipfail(c, e, "measure_copy");
+        // bytes read + bytes written per copy
+        *gbps = 2.0 * (double)n * sizeof(double) * reps / (ms * 1e-3) / 1e9;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_read_grid(icempm_ctx *c, double *out) {
-    if (!c || !out) return ICEMPM_E_ARG;
-    if (!c->gridA) return fail(c, ICEMPM_E_STATE, "read_grid: arrays not allocated");
-    HIPCHK(c, hipMemcpyAsync(out, c->gridA, c->gp * kGridRows * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream), "read_grid");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "read_grid");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !out) return ICEMPM_E_ARG;
+        if (!c->gridA) return fail(c, ICEMPM_E_STATE, "read_grid: arrays not allocated");
+        HIPCHK(c, hipMemcpyAsync(out, c->gridA, c->gp * kGridRows * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream), "read_grid");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "read_grid");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_write_grid(icempm_ctx *c, const double *in) {
-    if (!c || !in) return ICEMPM_E_ARG;
-    if (!c->gridA) return fail(c, ICEMPM_E_STATE, "write_grid: arrays not allocated");
-    HIPCHK(c, hipMemcpyAsync(c->gridA, in, c->gp * kGridRows * sizeof(double),
-                             hipMemcpyHostToDevice, c->stream), "write_grid");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "write_grid");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !in) return ICEMPM_E_ARG;
+        if (!c->gridA) return fail(c, ICEMPM_E_STATE, "write_grid: arrays not allocated");
+        HIPCHK(c, hipMemcpyAsync(c->gridA, in, c->gp * kGridRows * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream), "write_grid");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "write_grid");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_read_points(icempm_ctx *c, double *out) {
-    if (!c || !out) return ICEMPM_E_ARG;
-    if (!c->pts) return fail(c, ICEMPM_E_STATE, "read_points: arrays not allocated");
-    HIPCHK(c, hipMemcpyAsync(out, c->pts, c->pp * kPtsRows * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream), "read_points");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "read_points");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !out) return ICEMPM_E_ARG;
+        if (!c->pts) return fail(c, ICEMPM_E_STATE, "read_points: arrays not allocated");
+        HIPCHK(c, hipMemcpyAsync(out, c->pts, c->pp * kPtsRows * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream), "read_points");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "read_points");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_read_bins(icempm_ctx *c, double *out) {
-    if (!c || !out) return ICEMPM_E_ARG;
-    if (!c->bins) return fail(c, ICEMPM_E_STATE, "read_bins: arrays not allocated");
-    HIPCHK(c, hipMemcpyAsync(out, c->bins, (size_t)c->nBinsAlloc * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream), "read_bins");
-    HIPCHK(c, hipStreamSynchronize(c->stream), "read_bins");
-    return ICEMPM_OK;
+    return host_guard(c ? &c->lastError : nullptr, [&]() -> int {
+        if (!c || !out) return ICEMPM_E_ARG;
+        if (!c->bins) return fail(c, ICEMPM_E_STATE, "read_bins: arrays not allocated");
+        HIPCHK(c, hipMemcpyAsync(out, c->bins, (size_t)c->nBinsAlloc * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream), "read_bins");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "read_bins");
+        return ICEMPM_OK;
+    });
 }
 
 }  // extern "C"

@@ -1604,6 +1604,7 @@ void chunk_table(const icempm_batch *b, const unsigned *cells, std::vector<int> 
         // the calling thread works too; a helper that cannot be started
         // leaves its share to the others, and an exception (allocation)
         // in any of them is rethrown here, as the serial loop would throw
+        // (the entry point's host_guard turns it into ICEMPM_E_HOST)
         std::atomic<size_t> next{0};
         std::exception_ptr failure;
         std::mutex failureLock;
@@ -1883,25 +1884,27 @@ hipError_t enqueue_loop(icempm_batch *b, int n, const double *tTab, double *hist
 extern "C" {
 
 int icempm_batch_create(icempm_batch **out, int precision) {
-    if (!out || (precision != ICEMPM_BATCH_F64 && precision != ICEMPM_BATCH_F32)) return ICEMPM_E_ARG;
-    *out = nullptr;
-    icempm_batch *b = new icempm_batch();
-    b->precision = precision;
-    hipError_t e = hipSetDevice(0);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming);
-    for (int g = 0; g + 1 < icempm_batch::kChains && e == hipSuccess; ++g) {
-        e = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evJoin[g], hipEventDisableTiming);
-    }
-    if (e == hipSuccess) e = hipEventCreate(&b->e0);
-    if (e == hipSuccess) e = hipEventCreate(&b->e1);
-    if (e != hipSuccess) {
-        icempm_batch_destroy(b);
-        return ICEMPM_E_HIP;
-    }
-    *out = b;
-    return ICEMPM_OK;
+    return host_guard(nullptr, [&]() -> int {
+        if (!out || (precision != ICEMPM_BATCH_F64 && precision != ICEMPM_BATCH_F32)) return ICEMPM_E_ARG;
+        *out = nullptr;
+        icempm_batch *b = new icempm_batch();
+        b->precision = precision;
+        hipError_t e = hipSetDevice(0);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming);
+        for (int g = 0; g + 1 < icempm_batch::kChains && e == hipSuccess; ++g) {
+            e = hipStreamCreateWithFlags(&b->side[g], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evJoin[g], hipEventDisableTiming);
+        }
+        if (e == hipSuccess) e = hipEventCreate(&b->e0);
+        if (e == hipSuccess) e = hipEventCreate(&b->e1);
+        if (e != hipSuccess) {
+            icempm_batch_destroy(b);
+            return ICEMPM_E_HIP;
+        }
+        *out = b;
+        return ICEMPM_OK;
+    });
 }
 
 void icempm_batch_destroy(icempm_batch *b) {
@@ -1939,266 +1942,270 @@ const char *icempm_batch_last_error(const icempm_batch *b) { return b ? b->lastE
 
 int icempm_batch_add_case(icempm_batch *b, const void *simparams, size_t nbytes, const double *soa,
                           size_t pitch) {
-    if (!b || !simparams || !soa || nbytes != sizeof(icy::SimParams)) return -ICEMPM_E_ARG;
-    if (b->finalized) return -bfail(b, ICEMPM_E_STATE, "add_case: batch already finalized");
-    icy::SimParams P;
-    std::memcpy(&P, simparams, sizeof(P));
-    if (P.nPts <= 0 || (size_t)P.nPts > pitch || P.GridX <= 0 || P.GridY <= 0)
-        return -bfail(b, ICEMPM_E_ARG, "add_case: bad sizes");
-    if (P.GridX > 32767 || P.GridY > 32767)   // cell indices travel as 16-bit halves in the particle pass
-        return -bfail(b, ICEMPM_E_ARG, "add_case: grid wider or taller than 32767 cells");
-    // the case's grid slice, its node blocks and every node index the kernels
-    // form (X + Y * GridX) are sized from GridTotal: it must be the product
-    if ((long)P.GridTotal != (long)P.GridX * (long)P.GridY)
-        return -bfail(b, ICEMPM_E_ARG, "add_case: GridTotal is not GridX * GridY");
-    // the stencils use cellsize_inv, the weights and stresses cellsize and dt
-    // (derived as in ComputeHelperVariables, parameters_sim.cpp:131-133)
-    if (!(P.cellsize > 0) || !(P.InitialTimeStep > 0) || !(P.cellsize_inv == 1. / P.cellsize) ||
-        !(P.Dp_inv == 4. / (P.cellsize * P.cellsize)))
-        return -bfail(b, ICEMPM_E_ARG, "add_case: cellsize, cellsize_inv, Dp_inv and dt must be positive and consistent");
-    if (!(P.GrainVariability == P.GrainVariability))
-        return -bfail(b, ICEMPM_E_ARG, "add_case: GrainVariability is NaN");
-    if (P.n_indenter_subdivisions <= 0 || P.n_indenter_subdivisions > 1 << 16)
-        return -bfail(b, ICEMPM_E_ARG, "add_case: n_indenter_subdivisions out of range");
-    if (!b->hc.empty() && P.InitialTimeStep != b->hc[0].cp.dt)
-        return -bfail(b, ICEMPM_E_ARG, "add_case: all cases of a batch share one time step");
-    icempm_batch::HostCase h{};
-    CaseParams &c = h.cp;
-    c.dt = P.InitialTimeStep; c.vol = P.ParticleVolume; c.mass = P.ParticleMass; c.h = P.cellsize;
-    c.h_inv = P.cellsize_inv; c.Dinv = P.Dp_inv; c.mu = P.mu; c.kappa = P.kappa; c.gravity = P.Gravity;
-    c.indRsq = P.IndRSq; c.indV = P.IndVelocity; c.ics = P.IceCompressiveStrength;
-    c.its = P.IceTensileStrength; c.iss = P.IceShearStrength; c.tanPhi = P.DP_tan_phi;
-    c.dpThr = P.DP_threshold_p; c.x0 = P.indenter_x_initial; c.y0 = P.indenter_y;
-    {
-        // NACC constants of grain 0 with no variability (gpu_implementation5.cu:97-141)
-        const double pmax = P.IceCompressiveStrength, pmin = -P.IceTensileStrength, qmax = P.IceShearStrength;
-        const double beta = -pmin / pmax;
-        const double a = pmax * (1 + beta);
-        c.pmax = pmax;
-        c.msq = (4 * qmax * qmax * (1 + 2 * beta)) / (a * a);
-        c.betaPmax = beta * pmax;
-        c.onePlus2Beta = 1. + 2. * beta;
-        c.slope = -(2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin)) / -3e6;
-    }
-    c.gx = P.GridX; c.gy = P.GridY; c.gridTotal = P.GridTotal; c.nPts = P.nPts;
-    c.gxInv = 1.0 / c.gx;
-    c.grainVar = P.GrainVariability;
-    c.nbins = P.n_indenter_subdivisions;
-    c.binOffset = 0;   // set by finalize
-    c.scm = c.dt * c.vol * c.Dinv / c.mass;
-    {
-        // FP32 storage: 48 - ib fraction bits, ib the bits a cell index needs
-        // (finalize refuses grids beyond 65536 nodes a side: ib <= 16)
-        int ib = 0;
-        while (ib < 16 && (1 << ib) < std::max(c.gx, c.gy)) ++ib;
-        const int F = 48 - ib;
-        c.cellQ = std::ldexp(1.0, -F);
-        c.cellQ16 = std::ldexp(1.0, 16 - F);
-        c.cellInvQ = std::ldexp(1.0, F);
-        c.posQ = c.h * c.cellQ;
-        c.posInvQ = c.cellInvQ / c.h;
-    }
-    const int n = P.nPts;
-    h.pos.resize(2 * (size_t)n);
-    h.rest.resize(11 * (size_t)n);
-    h.crushed.resize(n);
-    h.grain.resize(n);
-    h.keys.resize(n);
-    const unsigned char *u8 = reinterpret_cast<const unsigned char *>(soa);
-    for (int i = 0; i < n; ++i) {
-        h.pos[i] = soa[pitch * 4 + i];
-        h.pos[n + i] = soa[pitch * 5 + i];
-        for (int r = 0; r < 10; ++r) h.rest[(size_t)r * n + i] = soa[pitch * (6 + r) + i];
-        h.rest[10 * (size_t)n + i] = soa[pitch * 3 + i];
-        h.crushed[i] = u8[i];
-        std::memcpy(&h.grain[i], u8 + pitch + 2 * (size_t)i, sizeof(short));   // point.cpp:19-23
-        // the base cell's band key, clamped into the grid as on the device
-        const double x = h.pos[i], y = h.pos[n + i];
-        int key = c.gridTotal - 1;
-        if (x == x && y == y) {
-            const double bx = std::fmin(std::fmax(x * c.h_inv - 0.5, 0.0), (double)(c.gx - 1));
-            const double by = std::fmin(std::fmax(y * c.h_inv - 0.5, 0.0), (double)(c.gy - 1));
-            key = band_key((int)bx, (int)by, c.gx, c.gy);
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !simparams || !soa || nbytes != sizeof(icy::SimParams)) return -ICEMPM_E_ARG;
+        if (b->finalized) return -bfail(b, ICEMPM_E_STATE, "add_case: batch already finalized");
+        icy::SimParams P;
+        std::memcpy(&P, simparams, sizeof(P));
+        if (P.nPts <= 0 || (size_t)P.nPts > pitch || P.GridX <= 0 || P.GridY <= 0)
+            return -bfail(b, ICEMPM_E_ARG, "add_case: bad sizes");
+        if (P.GridX > 32767 || P.GridY > 32767)   // cell indices travel as 16-bit halves in the particle pass
+            return -bfail(b, ICEMPM_E_ARG, "add_case: grid wider or taller than 32767 cells");
+        // the case's grid slice, its node blocks and every node index the kernels
+        // form (X + Y * GridX) are sized from GridTotal: it must be the product
+        if ((long)P.GridTotal != (long)P.GridX * (long)P.GridY)
+            return -bfail(b, ICEMPM_E_ARG, "add_case: GridTotal is not GridX * GridY");
+        // the stencils use cellsize_inv, the weights and stresses cellsize and dt
+        // (derived as in ComputeHelperVariables, parameters_sim.cpp:131-133)
+        if (!(P.cellsize > 0) || !(P.InitialTimeStep > 0) || !(P.cellsize_inv == 1. / P.cellsize) ||
+            !(P.Dp_inv == 4. / (P.cellsize * P.cellsize)))
+            return -bfail(b, ICEMPM_E_ARG, "add_case: cellsize, cellsize_inv, Dp_inv and dt must be positive and consistent");
+        if (!(P.GrainVariability == P.GrainVariability))
+            return -bfail(b, ICEMPM_E_ARG, "add_case: GrainVariability is NaN");
+        if (P.n_indenter_subdivisions <= 0 || P.n_indenter_subdivisions > 1 << 16)
+            return -bfail(b, ICEMPM_E_ARG, "add_case: n_indenter_subdivisions out of range");
+        if (!b->hc.empty() && P.InitialTimeStep != b->hc[0].cp.dt)
+            return -bfail(b, ICEMPM_E_ARG, "add_case: all cases of a batch share one time step");
+        icempm_batch::HostCase h{};
+        CaseParams &c = h.cp;
+        c.dt = P.InitialTimeStep; c.vol = P.ParticleVolume; c.mass = P.ParticleMass; c.h = P.cellsize;
+        c.h_inv = P.cellsize_inv; c.Dinv = P.Dp_inv; c.mu = P.mu; c.kappa = P.kappa; c.gravity = P.Gravity;
+        c.indRsq = P.IndRSq; c.indV = P.IndVelocity; c.ics = P.IceCompressiveStrength;
+        c.its = P.IceTensileStrength; c.iss = P.IceShearStrength; c.tanPhi = P.DP_tan_phi;
+        c.dpThr = P.DP_threshold_p; c.x0 = P.indenter_x_initial; c.y0 = P.indenter_y;
+        {
+            // NACC constants of grain 0 with no variability (gpu_implementation5.cu:97-141)
+            const double pmax = P.IceCompressiveStrength, pmin = -P.IceTensileStrength, qmax = P.IceShearStrength;
+            const double beta = -pmin / pmax;
+            const double a = pmax * (1 + beta);
+            c.pmax = pmax;
+            c.msq = (4 * qmax * qmax * (1 + 2 * beta)) / (a * a);
+            c.betaPmax = beta * pmax;
+            c.onePlus2Beta = 1. + 2. * beta;
+            c.slope = -(2 * std::sqrt(-pmax * pmin) * qmax / (pmax - pmin)) / -3e6;
         }
-        h.keys[i] = key;
-    }
-    b->hc.push_back(std::move(h));
-    return (int)b->hc.size() - 1;
+        c.gx = P.GridX; c.gy = P.GridY; c.gridTotal = P.GridTotal; c.nPts = P.nPts;
+        c.gxInv = 1.0 / c.gx;
+        c.grainVar = P.GrainVariability;
+        c.nbins = P.n_indenter_subdivisions;
+        c.binOffset = 0;   // set by finalize
+        c.scm = c.dt * c.vol * c.Dinv / c.mass;
+        {
+            // FP32 storage: 48 - ib fraction bits, ib the bits a cell index needs
+            // (finalize refuses grids beyond 65536 nodes a side: ib <= 16)
+            int ib = 0;
+            while (ib < 16 && (1 << ib) < std::max(c.gx, c.gy)) ++ib;
+            const int F = 48 - ib;
+            c.cellQ = std::ldexp(1.0, -F);
+            c.cellQ16 = std::ldexp(1.0, 16 - F);
+            c.cellInvQ = std::ldexp(1.0, F);
+            c.posQ = c.h * c.cellQ;
+            c.posInvQ = c.cellInvQ / c.h;
+        }
+        const int n = P.nPts;
+        h.pos.resize(2 * (size_t)n);
+        h.rest.resize(11 * (size_t)n);
+        h.crushed.resize(n);
+        h.grain.resize(n);
+        h.keys.resize(n);
+        const unsigned char *u8 = reinterpret_cast<const unsigned char *>(soa);
+        for (int i = 0; i < n; ++i) {
+            h.pos[i] = soa[pitch * 4 + i];
+            h.pos[n + i] = soa[pitch * 5 + i];
+            for (int r = 0; r < 10; ++r) h.rest[(size_t)r * n + i] = soa[pitch * (6 + r) + i];
+            h.rest[10 * (size_t)n + i] = soa[pitch * 3 + i];
+            h.crushed[i] = u8[i];
+            std::memcpy(&h.grain[i], u8 + pitch + 2 * (size_t)i, sizeof(short));   // point.cpp:19-23
+            // the base cell's band key, clamped into the grid as on the device
+            const double x = h.pos[i], y = h.pos[n + i];
+            int key = c.gridTotal - 1;
+            if (x == x && y == y) {
+                const double bx = std::fmin(std::fmax(x * c.h_inv - 0.5, 0.0), (double)(c.gx - 1));
+                const double by = std::fmin(std::fmax(y * c.h_inv - 0.5, 0.0), (double)(c.gy - 1));
+                key = band_key((int)bx, (int)by, c.gx, c.gy);
+            }
+            h.keys[i] = key;
+        }
+        b->hc.push_back(std::move(h));
+        return (int)b->hc.size() - 1;
+    });
 }
 
 int icempm_batch_finalize(icempm_batch *b) {
-    if (!b || b->hc.empty()) return ICEMPM_E_ARG;
-    if (b->finalized) return bfail(b, ICEMPM_E_STATE, "finalize: already finalized");
-    // point and node offsets, node-block table, keys of the points in slot order
-    std::vector<int2> nblk;
-    std::vector<int> keys;
-    long pofs = 0, gofs = 0, bofs = 0;
-    bool anyGrains = false;
-    // every case's points in band order (stable): slot o + k holds point
-    // order[k] of the case, and orig maps it back
-    std::vector<std::vector<int>> order(b->hc.size());
-    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
-        auto &h = b->hc[ci];
-        h.cp.ptOffset = (int)pofs;
-        h.cp.gridOffset = (int)gofs;
-        h.cp.binOffset = (int)bofs;
-        bofs += 2 * (long)h.cp.nbins;
-        anyGrains = anyGrains || h.cp.grainVar != 0;
-        auto &ord = order[ci];
-        ord.resize((size_t)h.cp.nPts);
-        for (int i = 0; i < h.cp.nPts; ++i) ord[(size_t)i] = i;
-        std::stable_sort(ord.begin(), ord.end(), [&h](int a, int c) { return h.keys[(size_t)a] < h.keys[(size_t)c]; });
-        for (int k = 0; k < h.cp.nPts; ++k) keys.push_back(h.keys[(size_t)ord[(size_t)k]]);
-        for (int nd = 0; nd < h.cp.gridTotal; nd += 256) nblk.push_back(make_int2((int)ci, nd));
-        pofs += h.cp.nPts;
-        gofs += h.cp.gridTotal;
-    }
-    // the particle pass addresses rows through 32-bit byte offsets
-    if (pofs >= (1L << 29) || gofs >= (1L << 29) || bofs > INT_MAX / 2)
-        return bfail(b, ICEMPM_E_ARG, "finalize: batch too large (2^29 points or grid nodes at most)");
-    // a base cell packs its column and row into 16 bits each
-    for (const auto &h : b->hc)
-        if (h.cp.gx > 65536 || h.cp.gy > 65536)
-            return bfail(b, ICEMPM_E_ARG, "finalize: a case's grid is wider or taller than 65536 nodes");
-    b->binsTotal = (int)bofs;
-    b->nPts = (int)pofs;
-    b->nNodes = gofs;
-    b->nNodeBlocks = (int)nblk.size();
-    // the groups of cases: group g starts at the case boundary nearest to
-    // g / chains of all points (each group non-empty)
-    {
-        const int nc = (int)b->hc.size();
-        b->chains = std::max(1, std::min(icempm_batch::kChains, nc));
-        b->groupCase[0] = 0;
-        long acc = 0;   // points of the cases before case k
-        int k = 0;
-        for (int g = 1; g < b->chains; ++g) {
-            const long target = (long)g * pofs;   // in units of 1 / chains points
-            if (k <= b->groupCase[g - 1]) {
-                acc += b->hc[k].cp.nPts;
-                ++k;
-            }
-            while (k < nc - (b->chains - g) &&
-                   std::labs((acc + b->hc[k].cp.nPts) * b->chains - target) <= std::labs(acc * b->chains - target)) {
-                acc += b->hc[k].cp.nPts;
-                ++k;
-            }
-            b->groupCase[g] = k;
-        }
-        b->groupCase[b->chains] = nc;
-        for (int g = 0; g <= b->chains; ++g) {
-            int kb = (int)nblk.size();
-            for (size_t q = 0; q < nblk.size(); ++q)
-                if (nblk[q].x >= b->groupCase[g]) {
-                    kb = (int)q;
-                    break;
-                }
-            b->grpBlock[g] = kb;
-        }
-    }
-    b->pitch = (pofs + 63) / 64 * 64;
-    b->gpitch = ((size_t)gofs + 63) / 64 * 64;
-    {
-        // the cells of the band keys, case-local (the key of a cell is
-        // band_key of its column and row)
-        std::vector<unsigned> cells(keys.size());
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || b->hc.empty()) return ICEMPM_E_ARG;
+        if (b->finalized) return bfail(b, ICEMPM_E_STATE, "finalize: already finalized");
+        // point and node offsets, node-block table, keys of the points in slot order
+        std::vector<int2> nblk;
+        std::vector<int> keys;
+        long pofs = 0, gofs = 0, bofs = 0;
+        bool anyGrains = false;
+        // every case's points in band order (stable): slot o + k holds point
+        // order[k] of the case, and orig maps it back
+        std::vector<std::vector<int>> order(b->hc.size());
         for (size_t ci = 0; ci < b->hc.size(); ++ci) {
-            const CaseParams &cp = b->hc[ci].cp;
-            for (int k = 0; k < cp.nPts; ++k) {
-                int cx, cy;
-                band_cell(keys[(size_t)cp.ptOffset + k], cp.gx, cp.gy, cx, cy);
-                cells[(size_t)cp.ptOffset + k] = (unsigned)cx | (unsigned)cy << 16;
+            auto &h = b->hc[ci];
+            h.cp.ptOffset = (int)pofs;
+            h.cp.gridOffset = (int)gofs;
+            h.cp.binOffset = (int)bofs;
+            bofs += 2 * (long)h.cp.nbins;
+            anyGrains = anyGrains || h.cp.grainVar != 0;
+            auto &ord = order[ci];
+            ord.resize((size_t)h.cp.nPts);
+            for (int i = 0; i < h.cp.nPts; ++i) ord[(size_t)i] = i;
+            std::stable_sort(ord.begin(), ord.end(), [&h](int a, int c) { return h.keys[(size_t)a] < h.keys[(size_t)c]; });
+            for (int k = 0; k < h.cp.nPts; ++k) keys.push_back(h.keys[(size_t)ord[(size_t)k]]);
+            for (int nd = 0; nd < h.cp.gridTotal; nd += 256) nblk.push_back(make_int2((int)ci, nd));
+            pofs += h.cp.nPts;
+            gofs += h.cp.gridTotal;
+        }
+        // the particle pass addresses rows through 32-bit byte offsets
+        if (pofs >= (1L << 29) || gofs >= (1L << 29) || bofs > INT_MAX / 2)
+            return bfail(b, ICEMPM_E_ARG, "finalize: batch too large (2^29 points or grid nodes at most)");
+        // a base cell packs its column and row into 16 bits each
+        for (const auto &h : b->hc)
+            if (h.cp.gx > 65536 || h.cp.gy > 65536)
+                return bfail(b, ICEMPM_E_ARG, "finalize: a case's grid is wider or taller than 65536 nodes");
+        b->binsTotal = (int)bofs;
+        b->nPts = (int)pofs;
+        b->nNodes = gofs;
+        b->nNodeBlocks = (int)nblk.size();
+        // the groups of cases: group g starts at the case boundary nearest to
+        // g / chains of all points (each group non-empty)
+        {
+            const int nc = (int)b->hc.size();
+            b->chains = std::max(1, std::min(icempm_batch::kChains, nc));
+            b->groupCase[0] = 0;
+            long acc = 0;   // points of the cases before case k
+            int k = 0;
+            for (int g = 1; g < b->chains; ++g) {
+                const long target = (long)g * pofs;   // in units of 1 / chains points
+                if (k <= b->groupCase[g - 1]) {
+                    acc += b->hc[k].cp.nPts;
+                    ++k;
+                }
+                while (k < nc - (b->chains - g) &&
+                       std::labs((acc + b->hc[k].cp.nPts) * b->chains - target) <= std::labs(acc * b->chains - target)) {
+                    acc += b->hc[k].cp.nPts;
+                    ++k;
+                }
+                b->groupCase[g] = k;
+            }
+            b->groupCase[b->chains] = nc;
+            for (int g = 0; g <= b->chains; ++g) {
+                int kb = (int)nblk.size();
+                for (size_t q = 0; q < nblk.size(); ++q)
+                    if (nblk[q].x >= b->groupCase[g]) {
+                        kb = (int)q;
+                        break;
+                    }
+                b->grpBlock[g] = kb;
             }
         }
-        chunk_table(b, cells.data(), b->hcstart, b->hccase);
-    }
-    const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
-    BCHK(b, hipMalloc(&b->pos, pos_bytes(b)), "batch finalize");
-    BCHK(b, hipMalloc(&b->rest, 11 * b->pitch * vs), "batch finalize");
-    BCHK(b, hipMalloc(&b->crushed, b->pitch), "batch finalize");
-    if (anyGrains) BCHK(b, hipMalloc(&b->grain, b->pitch * sizeof(short)), "batch finalize");
-    BCHK(b, hipMalloc(&b->bins, (size_t)b->binsTotal * sizeof(double)), "batch finalize");
-    BCHK(b, hipMalloc(&b->orig, b->pitch * sizeof(int)), "batch finalize");
-    BCHK(b, hipMalloc(&b->gridA, 3 * b->gpitch * sizeof(double)), "batch finalize");
-    BCHK(b, hipMalloc(&b->gridB, 3 * b->gpitch * sizeof(double)), "batch finalize");
-    if (b->precision == ICEMPM_BATCH_F32)
-        BCHK(b, hipMalloc(&b->vel32, b->gpitch * sizeof(float2)), "batch finalize");
-    BCHK(b, hipMalloc(&b->cases, b->hc.size() * sizeof(CaseParams)), "batch finalize");
-    BCHK(b, hipMalloc(&b->nodeBlock, nblk.size() * sizeof(int2)), "batch finalize");
-    BCHK(b, hipMalloc(&b->blkFlags, 3 * nblk.size() * sizeof(int)), "batch finalize");
-    BCHK(b, hipMalloc(&b->err, sizeof(int)), "batch finalize");
-    BCHK(b, hipMalloc(&b->hcount, sizeof(unsigned long long)), "batch finalize");
-    BCHK(b, hipMalloc(&b->stats, kMapStats * sizeof(unsigned long long)), "batch finalize");
-    // host staging in the device layout
-    std::vector<double> hpos(2 * b->pitch, 0.0);
-    std::vector<unsigned char> hrest(11 * b->pitch * vs, 0), hcr(b->pitch, 0);
-    std::vector<short> hgr(b->pitch, 0);
-    std::vector<int> horig(b->pitch, 0);
-    std::vector<CaseParams> hcp;
-    for (size_t ci = 0; ci < b->hc.size(); ++ci) {
-        const auto &h = b->hc[ci];
-        const int n = h.cp.nPts, o = h.cp.ptOffset;
-        for (int k = 0; k < n; ++k) {
-            const int i = order[ci][(size_t)k];
-            hpos[o + k] = h.pos[i];
-            hpos[b->pitch + o + k] = h.pos[n + i];
-            hcr[o + k] = h.crushed[i];
-            hgr[o + k] = h.grain[i];
-            horig[o + k] = o + i;
-            for (int r = 0; r < 11; ++r) {
-                double v = h.rest[(size_t)r * n + i];
-                if (b->precision == ICEMPM_BATCH_F32) {
-                    if (r == 2 || r == 5 || r == 10) v -= 1.0;    // Fe00, Fe11, Jp_inv stored less 1
-                    float f = (float)v;
-                    std::memcpy(&hrest[rest_elem(b, r, (size_t)(o + k)) * vs], &f, vs);
-                } else {
-                    std::memcpy(&hrest[rest_elem(b, r, (size_t)(o + k)) * vs], &v, vs);
+        b->pitch = (pofs + 63) / 64 * 64;
+        b->gpitch = ((size_t)gofs + 63) / 64 * 64;
+        {
+            // the cells of the band keys, case-local (the key of a cell is
+            // band_key of its column and row)
+            std::vector<unsigned> cells(keys.size());
+            for (size_t ci = 0; ci < b->hc.size(); ++ci) {
+                const CaseParams &cp = b->hc[ci].cp;
+                for (int k = 0; k < cp.nPts; ++k) {
+                    int cx, cy;
+                    band_cell(keys[(size_t)cp.ptOffset + k], cp.gx, cp.gy, cx, cy);
+                    cells[(size_t)cp.ptOffset + k] = (unsigned)cx | (unsigned)cy << 16;
                 }
             }
+            chunk_table(b, cells.data(), b->hcstart, b->hccase);
         }
-        hcp.push_back(h.cp);
-    }
-    if (b->precision == ICEMPM_BATCH_F32) {
-        // one record per slot: 48-bit fixed point (high words of x and y,
-        // both low halves) and the crushed flag
-        std::vector<unsigned int> q(4 * b->pitch, 0u);
-        for (size_t ci = 0; ci < hcp.size(); ++ci) {
-            const CaseParams &cp = hcp[ci];
-            for (int k = 0; k < cp.nPts; ++k) {
-                const size_t i = (size_t)cp.ptOffset + k;
-                unsigned int hx, lx, hy, ly;
-                fix48_put(hpos[i], cp, hx, lx);
-                fix48_put(hpos[b->pitch + i], cp, hy, ly);
-                q[4 * i] = hx;
-                q[4 * i + 1] = hy;
-                q[4 * i + 2] = lx | (ly << 16);
-                q[4 * i + 3] = hcr[i] | ((unsigned int)(unsigned short)hgr[i] << 16);
+        const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
+        BCHK(b, hipMalloc(&b->pos, pos_bytes(b)), "batch finalize");
+        BCHK(b, hipMalloc(&b->rest, 11 * b->pitch * vs), "batch finalize");
+        BCHK(b, hipMalloc(&b->crushed, b->pitch), "batch finalize");
+        if (anyGrains) BCHK(b, hipMalloc(&b->grain, b->pitch * sizeof(short)), "batch finalize");
+        BCHK(b, hipMalloc(&b->bins, (size_t)b->binsTotal * sizeof(double)), "batch finalize");
+        BCHK(b, hipMalloc(&b->orig, b->pitch * sizeof(int)), "batch finalize");
+        BCHK(b, hipMalloc(&b->gridA, 3 * b->gpitch * sizeof(double)), "batch finalize");
+        BCHK(b, hipMalloc(&b->gridB, 3 * b->gpitch * sizeof(double)), "batch finalize");
+        if (b->precision == ICEMPM_BATCH_F32)
+            BCHK(b, hipMalloc(&b->vel32, b->gpitch * sizeof(float2)), "batch finalize");
+        BCHK(b, hipMalloc(&b->cases, b->hc.size() * sizeof(CaseParams)), "batch finalize");
+        BCHK(b, hipMalloc(&b->nodeBlock, nblk.size() * sizeof(int2)), "batch finalize");
+        BCHK(b, hipMalloc(&b->blkFlags, 3 * nblk.size() * sizeof(int)), "batch finalize");
+        BCHK(b, hipMalloc(&b->err, sizeof(int)), "batch finalize");
+        BCHK(b, hipMalloc(&b->hcount, sizeof(unsigned long long)), "batch finalize");
+        BCHK(b, hipMalloc(&b->stats, kMapStats * sizeof(unsigned long long)), "batch finalize");
+        // host staging in the device layout
+        std::vector<double> hpos(2 * b->pitch, 0.0);
+        std::vector<unsigned char> hrest(11 * b->pitch * vs, 0), hcr(b->pitch, 0);
+        std::vector<short> hgr(b->pitch, 0);
+        std::vector<int> horig(b->pitch, 0);
+        std::vector<CaseParams> hcp;
+        for (size_t ci = 0; ci < b->hc.size(); ++ci) {
+            const auto &h = b->hc[ci];
+            const int n = h.cp.nPts, o = h.cp.ptOffset;
+            for (int k = 0; k < n; ++k) {
+                const int i = order[ci][(size_t)k];
+                hpos[o + k] = h.pos[i];
+                hpos[b->pitch + o + k] = h.pos[n + i];
+                hcr[o + k] = h.crushed[i];
+                hgr[o + k] = h.grain[i];
+                horig[o + k] = o + i;
+                for (int r = 0; r < 11; ++r) {
+                    double v = h.rest[(size_t)r * n + i];
+                    if (b->precision == ICEMPM_BATCH_F32) {
+                        if (r == 2 || r == 5 || r == 10) v -= 1.0;    // Fe00, Fe11, Jp_inv stored less 1
+                        float f = (float)v;
+                        std::memcpy(&hrest[rest_elem(b, r, (size_t)(o + k)) * vs], &f, vs);
+                    } else {
+                        std::memcpy(&hrest[rest_elem(b, r, (size_t)(o + k)) * vs], &v, vs);
+                    }
+                }
             }
+            hcp.push_back(h.cp);
         }
-        BCHK(b, hipMemcpy(b->pos, q.data(), q.size() * sizeof(unsigned int), hipMemcpyHostToDevice), "batch upload");
-    } else {
-        BCHK(b, hipMemcpy(b->pos, hpos.data(), hpos.size() * sizeof(double), hipMemcpyHostToDevice), "batch upload");
-    }
-    BCHK(b, hipMemcpy(b->rest, hrest.data(), hrest.size(), hipMemcpyHostToDevice), "batch upload");
-    BCHK(b, hipMemcpy(b->crushed, hcr.data(), hcr.size(), hipMemcpyHostToDevice), "batch upload");
-    if (b->grain)
-        BCHK(b, hipMemcpy(b->grain, hgr.data(), hgr.size() * sizeof(short), hipMemcpyHostToDevice), "batch upload");
-    {
-        const std::vector<double> zb((size_t)b->binsTotal, 0.0);
-        BCHK(b, hipMemcpy(b->bins, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice), "batch upload");
-    }
-    BCHK(b, hipMemcpy(b->orig, horig.data(), horig.size() * sizeof(int), hipMemcpyHostToDevice), "batch upload");
-    BCHK(b, hipMemcpy(b->cases, hcp.data(), hcp.size() * sizeof(CaseParams), hipMemcpyHostToDevice), "batch upload");
-    BCHK(b, hipMemcpy(b->nodeBlock, nblk.data(), nblk.size() * sizeof(int2), hipMemcpyHostToDevice), "batch upload");
-    int zero = 0;
-    BCHK(b, hipMemcpy(b->err, &zero, sizeof(int), hipMemcpyHostToDevice), "batch upload");
-    const unsigned long long z7[kMapStats] = {};
-    BCHK(b, hipMemcpy(b->hcount, z7, sizeof(unsigned long long), hipMemcpyHostToDevice), "batch upload");
-    BCHK(b, hipMemcpy(b->stats, z7, sizeof(z7), hipMemcpyHostToDevice), "batch upload");
-    int rc = upload_chunks(b);
-    if (rc) return rc;
-    b->finalized = true;
-    return ICEMPM_OK;
+        if (b->precision == ICEMPM_BATCH_F32) {
+            // one record per slot: 48-bit fixed point (high words of x and y,
+            // both low halves) and the crushed flag
+            std::vector<unsigned int> q(4 * b->pitch, 0u);
+            for (size_t ci = 0; ci < hcp.size(); ++ci) {
+                const CaseParams &cp = hcp[ci];
+                for (int k = 0; k < cp.nPts; ++k) {
+                    const size_t i = (size_t)cp.ptOffset + k;
+                    unsigned int hx, lx, hy, ly;
+                    fix48_put(hpos[i], cp, hx, lx);
+                    fix48_put(hpos[b->pitch + i], cp, hy, ly);
+                    q[4 * i] = hx;
+                    q[4 * i + 1] = hy;
+                    q[4 * i + 2] = lx | (ly << 16);
+                    q[4 * i + 3] = hcr[i] | ((unsigned int)(unsigned short)hgr[i] << 16);
+                }
+            }
+            BCHK(b, hipMemcpy(b->pos, q.data(), q.size() * sizeof(unsigned int), hipMemcpyHostToDevice), "batch upload");
+        } else {
+            BCHK(b, hipMemcpy(b->pos, hpos.data(), hpos.size() * sizeof(double), hipMemcpyHostToDevice), "batch upload");
+        }
+        BCHK(b, hipMemcpy(b->rest, hrest.data(), hrest.size(), hipMemcpyHostToDevice), "batch upload");
+        BCHK(b, hipMemcpy(b->crushed, hcr.data(), hcr.size(), hipMemcpyHostToDevice), "batch upload");
+        if (b->grain)
+            BCHK(b, hipMemcpy(b->grain, hgr.data(), hgr.size() * sizeof(short), hipMemcpyHostToDevice), "batch upload");
+        {
+            const std::vector<double> zb((size_t)b->binsTotal, 0.0);
+            BCHK(b, hipMemcpy(b->bins, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice), "batch upload");
+        }
+        BCHK(b, hipMemcpy(b->orig, horig.data(), horig.size() * sizeof(int), hipMemcpyHostToDevice), "batch upload");
+        BCHK(b, hipMemcpy(b->cases, hcp.data(), hcp.size() * sizeof(CaseParams), hipMemcpyHostToDevice), "batch upload");
+        BCHK(b, hipMemcpy(b->nodeBlock, nblk.data(), nblk.size() * sizeof(int2), hipMemcpyHostToDevice), "batch upload");
+        int zero = 0;
+        BCHK(b, hipMemcpy(b->err, &zero, sizeof(int), hipMemcpyHostToDevice), "batch upload");
+        const unsigned long long z7[kMapStats] = {};
+        BCHK(b, hipMemcpy(b->hcount, z7, sizeof(unsigned long long), hipMemcpyHostToDevice), "batch upload");
+        BCHK(b, hipMemcpy(b->stats, z7, sizeof(z7), hipMemcpyHostToDevice), "batch upload");
+        int rc = upload_chunks(b);
+        if (rc) return rc;
+        b->finalized = true;
+        return ICEMPM_OK;
+    });
 }
 
 }  // extern "C"
@@ -2299,61 +2306,65 @@ int run_segment(icempm_batch *b, int n, int use_graph, const double *tTab, doubl
 extern "C" {
 
 int icempm_batch_run(icempm_batch *b, int n, int use_graph) {
-    if (!b || n <= 0) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "run: batch not finalized");
-    if (use_graph && b->timingStride > 0)
-        return bfail(b, ICEMPM_E_STATE, "run: kernel timing needs stream launches (use_graph = 0)");
-    const size_t nc = b->hc.size();
-    int rc0 = ensure_capacity(b, n);
-    if (rc0) return rc0;
-    // the time of each substep by repeated addition (model.cpp:23)
-    std::vector<double> tt((size_t)n);
-    double t = b->t;
-    for (int i = 0; i < n; ++i) {
-        t += b->hc[0].cp.dt;
-        tt[i] = t;
-    }
-    BCHK(b, hipMemcpyAsync(b->tTab, tt.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream), "batch run");
-    // the force bins sum over the run (Model::Step sums them over a cycle, model.cpp:18)
-    BCHK(b, launch_fill_f64(b->bins, (size_t)b->binsTotal, 0.0, b->stream), "batch run");
-    BCHK(b, hipStreamSynchronize(b->stream), "batch run");
-    // segments of resortEvery substeps, the points re-sorted between them
-    const int seg = b->resortEvery > 0 ? b->resortEvery : n;
-    double total = 0;
-    for (int s0 = 0; s0 < n; s0 += seg) {
-        if (s0 > 0) {
-            const auto h0 = std::chrono::steady_clock::now();
-            int rc = resort(b);
-            if (rc) return rc;
-            total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || n <= 0) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "run: batch not finalized");
+        if (use_graph && b->timingStride > 0)
+            return bfail(b, ICEMPM_E_STATE, "run: kernel timing needs stream launches (use_graph = 0)");
+        const size_t nc = b->hc.size();
+        int rc0 = ensure_capacity(b, n);
+        if (rc0) return rc0;
+        // the time of each substep by repeated addition (model.cpp:23)
+        std::vector<double> tt((size_t)n);
+        double t = b->t;
+        for (int i = 0; i < n; ++i) {
+            t += b->hc[0].cp.dt;
+            tt[i] = t;
         }
-        const int len = std::min(seg, n - s0);
-        float ms = 0;
-        const auto h0 = std::chrono::steady_clock::now();
-        int rc = run_segment(b, len, use_graph, b->tTab + s0, b->hist + 2 * nc * (size_t)s0, &ms);
-        if (rc) return rc;
-        // with re-sorting, each segment is captured anew: count the host time
-        total += b->resortEvery > 0 ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count()
-                                    : (double)ms;
-    }
-    b->lastMs = (float)total;
-    b->t = t;
-    b->step += n;
-    b->lastRun = n;
-    return ICEMPM_OK;
+        BCHK(b, hipMemcpyAsync(b->tTab, tt.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream), "batch run");
+        // the force bins sum over the run (Model::Step sums them over a cycle, model.cpp:18)
+        BCHK(b, launch_fill_f64(b->bins, (size_t)b->binsTotal, 0.0, b->stream), "batch run");
+        BCHK(b, hipStreamSynchronize(b->stream), "batch run");
+        // segments of resortEvery substeps, the points re-sorted between them
+        const int seg = b->resortEvery > 0 ? b->resortEvery : n;
+        double total = 0;
+        for (int s0 = 0; s0 < n; s0 += seg) {
+            if (s0 > 0) {
+                const auto h0 = std::chrono::steady_clock::now();
+                int rc = resort(b);
+                if (rc) return rc;
+                total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+            }
+            const int len = std::min(seg, n - s0);
+            float ms = 0;
+            const auto h0 = std::chrono::steady_clock::now();
+            int rc = run_segment(b, len, use_graph, b->tTab + s0, b->hist + 2 * nc * (size_t)s0, &ms);
+            if (rc) return rc;
+            // with re-sorting, each segment is captured anew: count the host time
+            total += b->resortEvery > 0 ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count()
+                                        : (double)ms;
+        }
+        b->lastMs = (float)total;
+        b->t = t;
+        b->step += n;
+        b->lastRun = n;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_prepare(icempm_batch *b, int n) {
-    if (!b || n <= 0) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "prepare: batch not finalized");
-    BCHK(b, hipStreamSynchronize(b->stream), "prepare");
-    int rc = ensure_capacity(b, n);
-    if (rc) return rc;
-    // the first segment of such a run, from each of the three starts
-    const int len = b->resortEvery > 0 ? std::min(b->resortEvery, n) : n;
-    for (int start = -1; start <= 1; ++start)
-        if ((rc = ensure_graph(b, len, b->tTab, b->hist, start, nullptr))) return rc;
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || n <= 0) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "prepare: batch not finalized");
+        BCHK(b, hipStreamSynchronize(b->stream), "prepare");
+        int rc = ensure_capacity(b, n);
+        if (rc) return rc;
+        // the first segment of such a run, from each of the three starts
+        const int len = b->resortEvery > 0 ? std::min(b->resortEvery, n) : n;
+        for (int start = -1; start <= 1; ++start)
+            if ((rc = ensure_graph(b, len, b->tTab, b->hist, start, nullptr))) return rc;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_last_ms(const icempm_batch *b, float *ms) {
@@ -2367,217 +2378,235 @@ int icempm_batch_num_points(const icempm_batch *b) { return b ? b->nPts : 0; }
 long icempm_batch_num_nodes(const icempm_batch *b) { return b ? b->nNodes : 0; }
 
 int icempm_batch_kernel_timing(icempm_batch *b, int stride) {
-    if (!b || stride < 0) return ICEMPM_E_ARG;
-    b->timingStride = stride;
-    b->timedMs = 0;
-    b->timedLaunches = 0;
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || stride < 0) return ICEMPM_E_ARG;
+        b->timingStride = stride;
+        b->timedMs = 0;
+        b->timedLaunches = 0;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_kernel_timing_read(icempm_batch *b, double *total_ms, int *launches) {
-    if (!b || !total_ms || !launches) return ICEMPM_E_ARG;
-    *total_ms = b->timedMs;
-    *launches = b->timedLaunches;
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !total_ms || !launches) return ICEMPM_E_ARG;
+        *total_ms = b->timedMs;
+        *launches = b->timedLaunches;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_force_history(icempm_batch *b, int c, double *out, int n) {
-    if (!b || !out || c < 0 || (size_t)c >= b->hc.size() || n < 0 || n > b->lastRun) return ICEMPM_E_ARG;
-    const size_t nc = b->hc.size();
-    std::vector<double> all(2 * nc * (size_t)n);
-    if (n == 0) return ICEMPM_OK;
-    BCHK(b, hipMemcpy(all.data(), b->hist, all.size() * sizeof(double), hipMemcpyDeviceToHost), "force history");
-    for (int i = 0; i < n; ++i) {
-        out[2 * i] = all[(size_t)i * 2 * nc + 2 * c];
-        out[2 * i + 1] = all[(size_t)i * 2 * nc + 2 * c + 1];
-    }
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !out || c < 0 || (size_t)c >= b->hc.size() || n < 0 || n > b->lastRun) return ICEMPM_E_ARG;
+        const size_t nc = b->hc.size();
+        std::vector<double> all(2 * nc * (size_t)n);
+        if (n == 0) return ICEMPM_OK;
+        BCHK(b, hipMemcpy(all.data(), b->hist, all.size() * sizeof(double), hipMemcpyDeviceToHost), "force history");
+        for (int i = 0; i < n; ++i) {
+            out[2 * i] = all[(size_t)i * 2 * nc + 2 * c];
+            out[2 * i + 1] = all[(size_t)i * 2 * nc + 2 * c + 1];
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_force_bins(icempm_batch *b, int c, double *out, int nbins) {
-    if (!b || !out || c < 0 || (size_t)c >= b->hc.size()) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "force_bins: batch not finalized");
-    const CaseParams &cp = b->hc[c].cp;
-    if (nbins != cp.nbins) return bfail(b, ICEMPM_E_ARG, "force_bins: nbins is not the case's n_indenter_subdivisions");
-    BCHK(b, hipStreamSynchronize(b->stream), "force_bins");
-    BCHK(b, hipMemcpy(out, b->bins + cp.binOffset, 2 * (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost),
-         "force_bins");
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !out || c < 0 || (size_t)c >= b->hc.size()) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "force_bins: batch not finalized");
+        const CaseParams &cp = b->hc[c].cp;
+        if (nbins != cp.nbins) return bfail(b, ICEMPM_E_ARG, "force_bins: nbins is not the case's n_indenter_subdivisions");
+        BCHK(b, hipStreamSynchronize(b->stream), "force_bins");
+        BCHK(b, hipMemcpy(out, b->bins + cp.binOffset, 2 * (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost),
+             "force_bins");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_read_points(icempm_batch *b, int c, double *soa, size_t pitch) {
-    if (!b || !soa || c < 0 || (size_t)c >= b->hc.size()) return ICEMPM_E_ARG;
-    const CaseParams &cp = b->hc[c].cp;
-    const int n = cp.nPts, o = cp.ptOffset;
-    if ((size_t)n > pitch) return bfail(b, ICEMPM_E_ARG, "read_points: pitch too small");
-    const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
-    std::vector<double> hpos(2 * (size_t)n);
-    std::vector<unsigned char> hrest(11 * (size_t)n * vs), hcr(n);
-    std::vector<short> hgr((size_t)n, 0);
-    if (b->precision == ICEMPM_BATCH_F32) {
-        std::vector<unsigned int> q(4 * (size_t)n);
-        const unsigned int *P = static_cast<const unsigned int *>(b->pos);
-        BCHK(b, hipMemcpy(q.data(), P + 4 * (size_t)o, 4 * (size_t)n * sizeof(unsigned int), hipMemcpyDeviceToHost),
-             "read_points");
-        for (int i = 0; i < n; ++i) {
-            const unsigned int lo = q[4 * (size_t)i + 2];
-            hpos[i] = fix48_get(q[4 * (size_t)i], lo & 0xFFFFu, cp);
-            hpos[n + i] = fix48_get(q[4 * (size_t)i + 1], lo >> 16, cp);
-            hcr[i] = (unsigned char)(q[4 * (size_t)i + 3] & 0xFFu);
-            hgr[i] = (short)(q[4 * (size_t)i + 3] >> 16);
-        }
-    } else {
-        const double *P = static_cast<const double *>(b->pos);
-        BCHK(b, hipMemcpy(hpos.data(), P + o, n * sizeof(double), hipMemcpyDeviceToHost), "read_points");
-        BCHK(b, hipMemcpy(hpos.data() + n, P + b->pitch + o, n * sizeof(double), hipMemcpyDeviceToHost),
-             "read_points");
-    }
-    // rows of the case's slots [o, o + n) into hrest[r][slot]; FP32 storage's
-    // Fe records come down as one block and are split
-    for (int r = 0; r < 11; ++r) {
-        if (b->precision == ICEMPM_BATCH_F32 && r >= 2 && r <= 5) continue;
-        BCHK(b, hipMemcpy(hrest.data() + (size_t)r * n * vs, (unsigned char *)b->rest + rest_elem(b, r, o) * vs,
-                          n * vs, hipMemcpyDeviceToHost), "read_points");
-    }
-    if (b->precision == ICEMPM_BATCH_F32) {
-        std::vector<float> fe(4 * (size_t)n);
-        BCHK(b, hipMemcpy(fe.data(), (unsigned char *)b->rest + rest_elem(b, 2, o) * vs, 4 * (size_t)n * vs,
-                          hipMemcpyDeviceToHost), "read_points");
-        for (int slot = 0; slot < n; ++slot)
-            for (int r = 2; r <= 5; ++r)
-                std::memcpy(&hrest[((size_t)r * n + slot) * vs], &fe[4 * (size_t)slot + (r - 2)], vs);
-    }
-    if (b->precision != ICEMPM_BATCH_F32)   // FP32: from the position records above
-        BCHK(b, hipMemcpy(hcr.data(), b->crushed + o, n, hipMemcpyDeviceToHost), "read_points");
-    if (b->grain && b->precision != ICEMPM_BATCH_F32)
-        BCHK(b, hipMemcpy(hgr.data(), b->grain + o, n * sizeof(short), hipMemcpyDeviceToHost), "read_points");
-    std::vector<int> horig((size_t)n);
-    BCHK(b, hipMemcpy(horig.data(), b->orig + o, n * sizeof(int), hipMemcpyDeviceToHost), "read_points");
-    std::memset(soa, 0, 16 * pitch * sizeof(double));
-    unsigned char *u8 = reinterpret_cast<unsigned char *>(soa);
-    for (int slot = 0; slot < n; ++slot) {
-        // the point in this slot, at its index within the case
-        const int i = horig[slot] - o;
-        if (i < 0 || i >= n) return bfail(b, ICEMPM_E_STATE, "read_points: slot map corrupt");
-        u8[i] = hcr[slot];
-        std::memcpy(u8 + pitch + 2 * (size_t)i, &hgr[slot], sizeof(short));   // point.cpp:19-23
-        soa[pitch * 4 + i] = hpos[slot];
-        soa[pitch * 5 + i] = hpos[n + slot];
-        for (int r = 0; r < 11; ++r) {
-            double v;
-            if (vs == sizeof(float)) {
-                float f;
-                std::memcpy(&f, &hrest[((size_t)r * n + slot) * vs], vs);
-                v = f;
-                if (r == 2 || r == 5 || r == 10) v += 1.0;
-            } else {
-                std::memcpy(&v, &hrest[((size_t)r * n + slot) * vs], vs);
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !soa || c < 0 || (size_t)c >= b->hc.size()) return ICEMPM_E_ARG;
+        const CaseParams &cp = b->hc[c].cp;
+        const int n = cp.nPts, o = cp.ptOffset;
+        if ((size_t)n > pitch) return bfail(b, ICEMPM_E_ARG, "read_points: pitch too small");
+        const size_t vs = b->precision == ICEMPM_BATCH_F32 ? sizeof(float) : sizeof(double);
+        std::vector<double> hpos(2 * (size_t)n);
+        std::vector<unsigned char> hrest(11 * (size_t)n * vs), hcr(n);
+        std::vector<short> hgr((size_t)n, 0);
+        if (b->precision == ICEMPM_BATCH_F32) {
+            std::vector<unsigned int> q(4 * (size_t)n);
+            const unsigned int *P = static_cast<const unsigned int *>(b->pos);
+            BCHK(b, hipMemcpy(q.data(), P + 4 * (size_t)o, 4 * (size_t)n * sizeof(unsigned int), hipMemcpyDeviceToHost),
+                 "read_points");
+            for (int i = 0; i < n; ++i) {
+                const unsigned int lo = q[4 * (size_t)i + 2];
+                hpos[i] = fix48_get(q[4 * (size_t)i], lo & 0xFFFFu, cp);
+                hpos[n + i] = fix48_get(q[4 * (size_t)i + 1], lo >> 16, cp);
+                hcr[i] = (unsigned char)(q[4 * (size_t)i + 3] & 0xFFu);
+                hgr[i] = (short)(q[4 * (size_t)i + 3] >> 16);
             }
-            if (r < 10)
-                soa[pitch * (6 + r) + i] = v;
-            else
-                soa[pitch * 3 + i] = v;
+        } else {
+            const double *P = static_cast<const double *>(b->pos);
+            BCHK(b, hipMemcpy(hpos.data(), P + o, n * sizeof(double), hipMemcpyDeviceToHost), "read_points");
+            BCHK(b, hipMemcpy(hpos.data() + n, P + b->pitch + o, n * sizeof(double), hipMemcpyDeviceToHost),
+                 "read_points");
         }
-    }
-    return ICEMPM_OK;
+        // rows of the case's slots [o, o + n) into hrest[r][slot]; FP32 storage's
+        // Fe records come down as one block and are split
+        for (int r = 0; r < 11; ++r) {
+            if (b->precision == ICEMPM_BATCH_F32 && r >= 2 && r <= 5) continue;
+            BCHK(b, hipMemcpy(hrest.data() + (size_t)r * n * vs, (unsigned char *)b->rest + rest_elem(b, r, o) * vs,
+                              n * vs, hipMemcpyDeviceToHost), "read_points");
+        }
+        if (b->precision == ICEMPM_BATCH_F32) {
+            std::vector<float> fe(4 * (size_t)n);
+            BCHK(b, hipMemcpy(fe.data(), (unsigned char *)b->rest + rest_elem(b, 2, o) * vs, 4 * (size_t)n * vs,
+                              hipMemcpyDeviceToHost), "read_points");
+            for (int slot = 0; slot < n; ++slot)
+                for (int r = 2; r <= 5; ++r)
+                    std::memcpy(&hrest[((size_t)r * n + slot) * vs], &fe[4 * (size_t)slot + (r - 2)], vs);
+        }
+        if (b->precision != ICEMPM_BATCH_F32)   // FP32: from the position records above
+            BCHK(b, hipMemcpy(hcr.data(), b->crushed + o, n, hipMemcpyDeviceToHost), "read_points");
+        if (b->grain && b->precision != ICEMPM_BATCH_F32)
+            BCHK(b, hipMemcpy(hgr.data(), b->grain + o, n * sizeof(short), hipMemcpyDeviceToHost), "read_points");
+        std::vector<int> horig((size_t)n);
+        BCHK(b, hipMemcpy(horig.data(), b->orig + o, n * sizeof(int), hipMemcpyDeviceToHost), "read_points");
+        std::memset(soa, 0, 16 * pitch * sizeof(double));
+        unsigned char *u8 = reinterpret_cast<unsigned char *>(soa);
+        for (int slot = 0; slot < n; ++slot) {
+            // the point in this slot, at its index within the case
+            const int i = horig[slot] - o;
+            if (i < 0 || i >= n) return bfail(b, ICEMPM_E_STATE, "read_points: slot map corrupt");
+            u8[i] = hcr[slot];
+            std::memcpy(u8 + pitch + 2 * (size_t)i, &hgr[slot], sizeof(short));   // point.cpp:19-23
+            soa[pitch * 4 + i] = hpos[slot];
+            soa[pitch * 5 + i] = hpos[n + slot];
+            for (int r = 0; r < 11; ++r) {
+                double v;
+                if (vs == sizeof(float)) {
+                    float f;
+                    std::memcpy(&f, &hrest[((size_t)r * n + slot) * vs], vs);
+                    v = f;
+                    if (r == 2 || r == 5 || r == 10) v += 1.0;
+                } else {
+                    std::memcpy(&v, &hrest[((size_t)r * n + slot) * vs], vs);
+                }
+                if (r < 10)
+                    soa[pitch * (6 + r) + i] = v;
+                else
+                    soa[pitch * 3 + i] = v;
+            }
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_error_flag(icempm_batch *b, int *flag) {
-    if (!b || !flag) return ICEMPM_E_ARG;
-    BCHK(b, hipMemcpy(flag, b->err, sizeof(int), hipMemcpyDeviceToHost), "error flag");
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !flag) return ICEMPM_E_ARG;
+        BCHK(b, hipMemcpy(flag, b->err, sizeof(int), hipMemcpyDeviceToHost), "error flag");
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_set_return_map(icempm_batch *b, int mode, const float *params, const float *norm,
                                 double tol, int harvest_capacity) {
-    if (!b) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "set_return_map: batch not finalized");
-    if (mode != kMapExact && mode != kMapHarvest && mode != kMapLearned)
-        return bfail(b, ICEMPM_E_ARG, "set_return_map: unknown mode");
-    if (mode == kMapHarvest && harvest_capacity <= 0)
-        return bfail(b, ICEMPM_E_ARG, "set_return_map: harvest needs a positive capacity");
-    if (mode == kMapLearned && (!params || !norm))
-        return bfail(b, ICEMPM_E_ARG, "set_return_map: the learned map needs parameters and a normalisation");
-    if (mode == kMapLearned && !(tol == tol))
-        return bfail(b, ICEMPM_E_ARG, "set_return_map: tolerance is NaN");
-    BCHK(b, hipStreamSynchronize(b->stream), "set_return_map");
-    // a captured loop holds the previous kernel arguments
-    drop_graph(b);
-    if (mode == kMapHarvest) {
-        if (harvest_capacity != b->hcap) {
-            dfree(b->hx);
-            dfree(b->ht);
-            b->hcap = 0;
-            BCHK(b, hipMalloc(&b->hx, (size_t)harvest_capacity * 5 * sizeof(float)), "set_return_map");
-            BCHK(b, hipMalloc(&b->ht, (size_t)harvest_capacity * 3 * sizeof(float)), "set_return_map");
-            b->hcap = harvest_capacity;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "set_return_map: batch not finalized");
+        if (mode != kMapExact && mode != kMapHarvest && mode != kMapLearned)
+            return bfail(b, ICEMPM_E_ARG, "set_return_map: unknown mode");
+        if (mode == kMapHarvest && harvest_capacity <= 0)
+            return bfail(b, ICEMPM_E_ARG, "set_return_map: harvest needs a positive capacity");
+        if (mode == kMapLearned && (!params || !norm))
+            return bfail(b, ICEMPM_E_ARG, "set_return_map: the learned map needs parameters and a normalisation");
+        if (mode == kMapLearned && !(tol == tol))
+            return bfail(b, ICEMPM_E_ARG, "set_return_map: tolerance is NaN");
+        BCHK(b, hipStreamSynchronize(b->stream), "set_return_map");
+        // a captured loop holds the previous kernel arguments
+        drop_graph(b);
+        if (mode == kMapHarvest) {
+            if (harvest_capacity != b->hcap) {
+                dfree(b->hx);
+                dfree(b->ht);
+                b->hcap = 0;
+                BCHK(b, hipMalloc(&b->hx, (size_t)harvest_capacity * 5 * sizeof(float)), "set_return_map");
+                BCHK(b, hipMalloc(&b->ht, (size_t)harvest_capacity * 3 * sizeof(float)), "set_return_map");
+                b->hcap = harvest_capacity;
+            }
+            const unsigned long long zero = 0;
+            BCHK(b, hipMemcpy(b->hcount, &zero, sizeof(zero), hipMemcpyHostToDevice), "set_return_map");
         }
-        const unsigned long long zero = 0;
-        BCHK(b, hipMemcpy(b->hcount, &zero, sizeof(zero), hipMemcpyHostToDevice), "set_return_map");
-    }
-    if (mode == kMapLearned) {
-        const int np = mlpdev::kParams;
-        std::vector<unsigned short> wb(np);
-        for (int i = 0; i < np; ++i) wb[i] = mlpdev::to_bf16(params[i]);
-        if (!b->mlpW) BCHK(b, hipMalloc(&b->mlpW, np * sizeof(unsigned short)), "set_return_map");
-        if (!b->mlpP) BCHK(b, hipMalloc(&b->mlpP, np * sizeof(float)), "set_return_map");
-        BCHK(b, hipMemcpy(b->mlpW, wb.data(), np * sizeof(unsigned short), hipMemcpyHostToDevice), "set_return_map");
-        BCHK(b, hipMemcpy(b->mlpP, params, np * sizeof(float), hipMemcpyHostToDevice), "set_return_map");
-        std::memcpy(b->norm.in_mean, norm, 5 * sizeof(float));
-        std::memcpy(b->norm.in_scale, norm + 5, 5 * sizeof(float));
-        std::memcpy(b->norm.out_mean, norm + 10, 3 * sizeof(float));
-        std::memcpy(b->norm.out_scale, norm + 13, 3 * sizeof(float));
-        for (int q = 0; q < 5; ++q)
-            if (!(b->norm.in_scale[q] > 0)) return bfail(b, ICEMPM_E_ARG, "set_return_map: input scale must be > 0");
-        b->tol = tol;
-        if (!b->mapQ) BCHK(b, hipMalloc(&b->mapQ, (size_t)b->nPts * sizeof(MapEntry)), "set_return_map");
-        if (!b->mapCount) BCHK(b, hipMalloc(&b->mapCount, 2 * sizeof(unsigned int)), "set_return_map");
-    }
-    b->rmap = mode;
-    return ICEMPM_OK;
+        if (mode == kMapLearned) {
+            const int np = mlpdev::kParams;
+            std::vector<unsigned short> wb(np);
+            for (int i = 0; i < np; ++i) wb[i] = mlpdev::to_bf16(params[i]);
+            if (!b->mlpW) BCHK(b, hipMalloc(&b->mlpW, np * sizeof(unsigned short)), "set_return_map");
+            if (!b->mlpP) BCHK(b, hipMalloc(&b->mlpP, np * sizeof(float)), "set_return_map");
+            BCHK(b, hipMemcpy(b->mlpW, wb.data(), np * sizeof(unsigned short), hipMemcpyHostToDevice), "set_return_map");
+            BCHK(b, hipMemcpy(b->mlpP, params, np * sizeof(float), hipMemcpyHostToDevice), "set_return_map");
+            std::memcpy(b->norm.in_mean, norm, 5 * sizeof(float));
+            std::memcpy(b->norm.in_scale, norm + 5, 5 * sizeof(float));
+            std::memcpy(b->norm.out_mean, norm + 10, 3 * sizeof(float));
+            std::memcpy(b->norm.out_scale, norm + 13, 3 * sizeof(float));
+            for (int q = 0; q < 5; ++q)
+                if (!(b->norm.in_scale[q] > 0)) return bfail(b, ICEMPM_E_ARG, "set_return_map: input scale must be > 0");
+            b->tol = tol;
+            if (!b->mapQ) BCHK(b, hipMalloc(&b->mapQ, (size_t)b->nPts * sizeof(MapEntry)), "set_return_map");
+            if (!b->mapCount) BCHK(b, hipMalloc(&b->mapCount, 2 * sizeof(unsigned int)), "set_return_map");
+        }
+        b->rmap = mode;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_read_harvest(icempm_batch *b, float *X, float *T, int cap, unsigned long long *total) {
-    if (!b || !total || cap < 0 || (cap > 0 && (!X || !T))) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "read_harvest: batch not finalized");
-    BCHK(b, hipStreamSynchronize(b->stream), "read_harvest");
-    unsigned long long n = 0;
-    BCHK(b, hipMemcpy(&n, b->hcount, sizeof(n), hipMemcpyDeviceToHost), "read_harvest");
-    *total = n;
-    const int m = (int)std::min<unsigned long long>(std::min<unsigned long long>(n, (unsigned long long)b->hcap),
-                                                    (unsigned long long)cap);
-    if (m > 0) {
-        BCHK(b, hipMemcpy(X, b->hx, (size_t)m * 5 * sizeof(float), hipMemcpyDeviceToHost), "read_harvest");
-        BCHK(b, hipMemcpy(T, b->ht, (size_t)m * 3 * sizeof(float), hipMemcpyDeviceToHost), "read_harvest");
-    }
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !total || cap < 0 || (cap > 0 && (!X || !T))) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "read_harvest: batch not finalized");
+        BCHK(b, hipStreamSynchronize(b->stream), "read_harvest");
+        unsigned long long n = 0;
+        BCHK(b, hipMemcpy(&n, b->hcount, sizeof(n), hipMemcpyDeviceToHost), "read_harvest");
+        *total = n;
+        const int m = (int)std::min<unsigned long long>(std::min<unsigned long long>(n, (unsigned long long)b->hcap),
+                                                        (unsigned long long)cap);
+        if (m > 0) {
+            BCHK(b, hipMemcpy(X, b->hx, (size_t)m * 5 * sizeof(float), hipMemcpyDeviceToHost), "read_harvest");
+            BCHK(b, hipMemcpy(T, b->ht, (size_t)m * 3 * sizeof(float), hipMemcpyDeviceToHost), "read_harvest");
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_set_tuning(icempm_batch *b, int knob, int value) {
-    if (!b) return ICEMPM_E_ARG;
-    switch (knob) {
-    case ICEMPM_BATCH_TUNE_CHUNK:
-        if (value != 64 && value != 128 && value != 256)
-            return bfail(b, ICEMPM_E_ARG, "set_tuning: points per chunk is 64, 128 or 256");
-        if (value != b->chunkPoints) {
-            b->chunkPoints = value;
-            if (b->finalized) {
-                BCHK(b, hipStreamSynchronize(b->stream), "set_tuning");
-                return rebuild_chunks(b);
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b) return ICEMPM_E_ARG;
+        switch (knob) {
+        case ICEMPM_BATCH_TUNE_CHUNK:
+            if (value != 64 && value != 128 && value != 256)
+                return bfail(b, ICEMPM_E_ARG, "set_tuning: points per chunk is 64, 128 or 256");
+            if (value != b->chunkPoints) {
+                b->chunkPoints = value;
+                if (b->finalized) {
+                    BCHK(b, hipStreamSynchronize(b->stream), "set_tuning");
+                    return rebuild_chunks(b);
+                }
             }
+            return ICEMPM_OK;
+        case ICEMPM_BATCH_TUNE_RESORT:
+            if (value < 0) return bfail(b, ICEMPM_E_ARG, "set_tuning: the re-sort cadence is >= 0");
+            b->resortEvery = value;
+            return ICEMPM_OK;
+        case ICEMPM_BATCH_TUNE_MAP_MIN_ACCEPT:
+            if (value < 0 || value > 100) return bfail(b, ICEMPM_E_ARG, "set_tuning: acceptance threshold is 0..100 percent");
+            b->mapMinAccept = value / 100.0;
+            return ICEMPM_OK;
+        default:
+            return bfail(b, ICEMPM_E_ARG, "set_tuning: unknown knob");
         }
-        return ICEMPM_OK;
-    case ICEMPM_BATCH_TUNE_RESORT:
-        if (value < 0) return bfail(b, ICEMPM_E_ARG, "set_tuning: the re-sort cadence is >= 0");
-        b->resortEvery = value;
-        return ICEMPM_OK;
-    case ICEMPM_BATCH_TUNE_MAP_MIN_ACCEPT:
-        if (value < 0 || value > 100) return bfail(b, ICEMPM_E_ARG, "set_tuning: acceptance threshold is 0..100 percent");
-        b->mapMinAccept = value / 100.0;
-        return ICEMPM_OK;
-    default:
-        return bfail(b, ICEMPM_E_ARG, "set_tuning: unknown knob");
-    }
+    });
 }
 
 int icempm_batch_get_tuning(const icempm_batch *b, int knob) {
@@ -2591,174 +2620,188 @@ int icempm_batch_get_tuning(const icempm_batch *b, int knob) {
 }
 
 int icempm_batch_resort(icempm_batch *b) {
-    if (!b) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "resort: batch not finalized");
-    BCHK(b, hipStreamSynchronize(b->stream), "resort");
-    return resort(b);
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "resort: batch not finalized");
+        BCHK(b, hipStreamSynchronize(b->stream), "resort");
+        return resort(b);
+    });
 }
 
 int icempm_batch_checkpoint(icempm_batch *b) {
-    if (!b) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "checkpoint: batch not finalized");
-    if (!b->kpos) {
-        BCHK(b, hipMalloc(&b->kpos, pos_bytes(b)), "checkpoint");
-        BCHK(b, hipMalloc(&b->krest, rest_bytes(b)), "checkpoint");
-        BCHK(b, hipMalloc(&b->kcrushed, b->pitch), "checkpoint");
-        BCHK(b, hipMalloc(&b->korig, b->pitch * sizeof(int)), "checkpoint");
-        if (b->grain) BCHK(b, hipMalloc(&b->kgrain, b->pitch * sizeof(short)), "checkpoint");
-    }
-    BCHK(b, hipMemcpyAsync(b->kpos, b->pos, pos_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
-    BCHK(b, hipMemcpyAsync(b->krest, b->rest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
-    BCHK(b, hipMemcpyAsync(b->kcrushed, b->crushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "checkpoint");
-    BCHK(b, hipMemcpyAsync(b->korig, b->orig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
-    if (b->grain)   // re-sorts permute the grain row with the points
-        BCHK(b, hipMemcpyAsync(b->kgrain, b->grain, b->pitch * sizeof(short), hipMemcpyDeviceToDevice, b->stream),
-             "checkpoint");
-    BCHK(b, hipStreamSynchronize(b->stream), "checkpoint");
-    b->kt = b->t;
-    b->kstep = b->step;
-    b->kcstart = b->hcstart;
-    b->kccase = b->hccase;
-    b->kChunkPoints = b->chunkPoints;
-    b->haveCheckpoint = true;
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "checkpoint: batch not finalized");
+        if (!b->kpos) {
+            BCHK(b, hipMalloc(&b->kpos, pos_bytes(b)), "checkpoint");
+            BCHK(b, hipMalloc(&b->krest, rest_bytes(b)), "checkpoint");
+            BCHK(b, hipMalloc(&b->kcrushed, b->pitch), "checkpoint");
+            BCHK(b, hipMalloc(&b->korig, b->pitch * sizeof(int)), "checkpoint");
+            if (b->grain) BCHK(b, hipMalloc(&b->kgrain, b->pitch * sizeof(short)), "checkpoint");
+        }
+        BCHK(b, hipMemcpyAsync(b->kpos, b->pos, pos_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+        BCHK(b, hipMemcpyAsync(b->krest, b->rest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+        BCHK(b, hipMemcpyAsync(b->kcrushed, b->crushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+        BCHK(b, hipMemcpyAsync(b->korig, b->orig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "checkpoint");
+        if (b->grain)   // re-sorts permute the grain row with the points
+            BCHK(b, hipMemcpyAsync(b->kgrain, b->grain, b->pitch * sizeof(short), hipMemcpyDeviceToDevice, b->stream),
+                 "checkpoint");
+        BCHK(b, hipStreamSynchronize(b->stream), "checkpoint");
+        b->kt = b->t;
+        b->kstep = b->step;
+        b->kcstart = b->hcstart;
+        b->kccase = b->hccase;
+        b->kChunkPoints = b->chunkPoints;
+        b->haveCheckpoint = true;
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_restore(icempm_batch *b) {
-    if (!b) return ICEMPM_E_ARG;
-    if (!b->haveCheckpoint) return bfail(b, ICEMPM_E_STATE, "restore: no checkpoint");
-    BCHK(b, hipStreamSynchronize(b->stream), "restore");
-    BCHK(b, hipMemcpyAsync(b->pos, b->kpos, pos_bytes(b), hipMemcpyDeviceToDevice, b->stream), "restore");
-    BCHK(b, hipMemcpyAsync(b->rest, b->krest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "restore");
-    BCHK(b, hipMemcpyAsync(b->crushed, b->kcrushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "restore");
-    BCHK(b, hipMemcpyAsync(b->orig, b->korig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "restore");
-    if (b->grain)
-        BCHK(b, hipMemcpyAsync(b->grain, b->kgrain, b->pitch * sizeof(short), hipMemcpyDeviceToDevice, b->stream),
-             "restore");
-    BCHK(b, hipStreamSynchronize(b->stream), "restore");
-    b->pending = -1;   // the grids hold the P2G of the state before the restore
-    b->t = b->kt;
-    b->step = b->kstep;
-    b->hcstart = b->kcstart;
-    b->hccase = b->kccase;
-    b->chunkPoints = b->kChunkPoints;
-    return upload_chunks(b);
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b) return ICEMPM_E_ARG;
+        if (!b->haveCheckpoint) return bfail(b, ICEMPM_E_STATE, "restore: no checkpoint");
+        BCHK(b, hipStreamSynchronize(b->stream), "restore");
+        BCHK(b, hipMemcpyAsync(b->pos, b->kpos, pos_bytes(b), hipMemcpyDeviceToDevice, b->stream), "restore");
+        BCHK(b, hipMemcpyAsync(b->rest, b->krest, rest_bytes(b), hipMemcpyDeviceToDevice, b->stream), "restore");
+        BCHK(b, hipMemcpyAsync(b->crushed, b->kcrushed, b->pitch, hipMemcpyDeviceToDevice, b->stream), "restore");
+        BCHK(b, hipMemcpyAsync(b->orig, b->korig, b->pitch * sizeof(int), hipMemcpyDeviceToDevice, b->stream), "restore");
+        if (b->grain)
+            BCHK(b, hipMemcpyAsync(b->grain, b->kgrain, b->pitch * sizeof(short), hipMemcpyDeviceToDevice, b->stream),
+                 "restore");
+        BCHK(b, hipStreamSynchronize(b->stream), "restore");
+        b->pending = -1;   // the grids hold the P2G of the state before the restore
+        b->t = b->kt;
+        b->step = b->kstep;
+        b->hcstart = b->kcstart;
+        b->hccase = b->kccase;
+        b->chunkPoints = b->kChunkPoints;
+        return upload_chunks(b);
+    });
 }
 
 int icempm_batch_autotune(icempm_batch *b, int probe_steps, double *report, int cap, int *rows) {
-    if (!b || probe_steps < 16 || cap < 0 || (cap > 0 && !report) || !rows) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "autotune: batch not finalized");
-    if (b->timingStride > 0) return bfail(b, ICEMPM_E_STATE, "autotune: switch kernel timing off first");
-    // the probes would append their pairs to the harvest buffer
-    if (b->rmap == kMapHarvest) return bfail(b, ICEMPM_E_STATE, "autotune: not in harvest mode");
-    int rc = icempm_batch_checkpoint(b);
-    if (rc) return rc;
-    // what the probes change besides the point state: the map statistics
-    // and the error flag (restored below), the force history and lastRun
-    // (the probes' own; a caller reads them after its next run)
-    unsigned long long stats0[kMapStats] = {};
-    int err0 = 0;
-    BCHK(b, hipMemcpy(stats0, b->stats, sizeof(stats0), hipMemcpyDeviceToHost), "autotune");
-    BCHK(b, hipMemcpy(&err0, b->err, sizeof(int), hipMemcpyDeviceToHost), "autotune");
-    const int chunks[3] = {256, 128, 64};
-    const int resorts[3] = {0, probe_steps / 2, probe_steps / 4};
-    double best = 1e300;
-    int bestChunk = b->chunkPoints, bestResort = b->resortEvery, nrow = 0;
-    for (int ci = 0; ci < 3; ++ci)
-        for (int ri = 0; ri < 3; ++ri) {
-            // one clock for every candidate: the host time of the whole
-            // probe (re-sorts and graph captures included), measured on a
-            // second run so that the no-re-sort candidates replay a graph
-            // captured by the first
-            b->chunkPoints = chunks[ci];
-            b->resortEvery = resorts[ri];
-            double per = 0;
-            for (int pass = 0; pass < 2; ++pass) {
-                if ((rc = icempm_batch_restore(b))) return rc;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || probe_steps < 16 || cap < 0 || (cap > 0 && !report) || !rows) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "autotune: batch not finalized");
+        if (b->timingStride > 0) return bfail(b, ICEMPM_E_STATE, "autotune: switch kernel timing off first");
+        // the probes would append their pairs to the harvest buffer
+        if (b->rmap == kMapHarvest) return bfail(b, ICEMPM_E_STATE, "autotune: not in harvest mode");
+        int rc = icempm_batch_checkpoint(b);
+        if (rc) return rc;
+        // what the probes change besides the point state: the map statistics
+        // and the error flag (restored below), the force history and lastRun
+        // (the probes' own; a caller reads them after its next run)
+        unsigned long long stats0[kMapStats] = {};
+        int err0 = 0;
+        BCHK(b, hipMemcpy(stats0, b->stats, sizeof(stats0), hipMemcpyDeviceToHost), "autotune");
+        BCHK(b, hipMemcpy(&err0, b->err, sizeof(int), hipMemcpyDeviceToHost), "autotune");
+        const int chunks[3] = {256, 128, 64};
+        const int resorts[3] = {0, probe_steps / 2, probe_steps / 4};
+        double best = 1e300;
+        int bestChunk = b->chunkPoints, bestResort = b->resortEvery, nrow = 0;
+        for (int ci = 0; ci < 3; ++ci)
+            for (int ri = 0; ri < 3; ++ri) {
+                // one clock for every candidate: the host time of the whole
+                // probe (re-sorts and graph captures included), measured on a
+                // second run so that the no-re-sort candidates replay a graph
+                // captured by the first
                 b->chunkPoints = chunks[ci];
-                if ((rc = rebuild_chunks(b))) return rc;
-                const auto h0 = std::chrono::steady_clock::now();
-                if ((rc = icempm_batch_run(b, probe_steps, 1))) return rc;
-                per = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count() /
-                      probe_steps;
+                b->resortEvery = resorts[ri];
+                double per = 0;
+                for (int pass = 0; pass < 2; ++pass) {
+                    if ((rc = icempm_batch_restore(b))) return rc;
+                    b->chunkPoints = chunks[ci];
+                    if ((rc = rebuild_chunks(b))) return rc;
+                    const auto h0 = std::chrono::steady_clock::now();
+                    if ((rc = icempm_batch_run(b, probe_steps, 1))) return rc;
+                    per = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count() /
+                          probe_steps;
+                }
+                if (nrow < cap) {
+                    report[3 * nrow] = chunks[ci];
+                    report[3 * nrow + 1] = resorts[ri];
+                    report[3 * nrow + 2] = per;
+                }
+                ++nrow;
+                if (per < best) {
+                    best = per;
+                    bestChunk = chunks[ci];
+                    bestResort = resorts[ri];
+                }
             }
-            if (nrow < cap) {
-                report[3 * nrow] = chunks[ci];
-                report[3 * nrow + 1] = resorts[ri];
-                report[3 * nrow + 2] = per;
-            }
-            ++nrow;
-            if (per < best) {
-                best = per;
-                bestChunk = chunks[ci];
-                bestResort = resorts[ri];
-            }
+        *rows = nrow;
+        // With the learned map: a branch whose network proposals the yield gate
+        // passed less often than mapMinAccept over the probes is switched off
+        // (its projections take the exact map in the particle pass, with no
+        // queue and no map kernel)
+        if (b->rmap == kMapLearned) {
+            unsigned long long st[kMapStats];
+            BCHK(b, hipMemcpy(st, b->stats, sizeof(st), hipMemcpyDeviceToHost), "autotune");
+            int mask = 0;
+            const unsigned long long tA = st[3] - stats0[3], tF = st[4] - stats0[4];
+            const unsigned long long sA = st[5] - stats0[5], sF = st[6] - stats0[6];
+            if ((b->mapBranches & kMapTension) && tA + tF > 0 && (double)tA >= b->mapMinAccept * (double)(tA + tF))
+                mask |= kMapTension;
+            if ((b->mapBranches & kMapShear) && sA + sF > 0 && (double)sA >= b->mapMinAccept * (double)(sA + sF))
+                mask |= kMapShear;
+            b->tuneBranchStats[0] = tA; b->tuneBranchStats[1] = tF;
+            b->tuneBranchStats[2] = sA; b->tuneBranchStats[3] = sF;
+            if ((rc = icempm_batch_set_map_branches(b, mask))) return rc;
         }
-    *rows = nrow;
-    // With the learned map: a branch whose network proposals the yield gate
-    // passed less often than mapMinAccept over the probes is switched off
-    // (its projections take the exact map in the particle pass, with no
-    // queue and no map kernel)
-    if (b->rmap == kMapLearned) {
-        unsigned long long st[kMapStats];
-        BCHK(b, hipMemcpy(st, b->stats, sizeof(st), hipMemcpyDeviceToHost), "autotune");
-        int mask = 0;
-        const unsigned long long tA = st[3] - stats0[3], tF = st[4] - stats0[4];
-        const unsigned long long sA = st[5] - stats0[5], sF = st[6] - stats0[6];
-        if ((b->mapBranches & kMapTension) && tA + tF > 0 && (double)tA >= b->mapMinAccept * (double)(tA + tF))
-            mask |= kMapTension;
-        if ((b->mapBranches & kMapShear) && sA + sF > 0 && (double)sA >= b->mapMinAccept * (double)(sA + sF))
-            mask |= kMapShear;
-        b->tuneBranchStats[0] = tA; b->tuneBranchStats[1] = tF;
-        b->tuneBranchStats[2] = sA; b->tuneBranchStats[3] = sF;
-        if ((rc = icempm_batch_set_map_branches(b, mask))) return rc;
-    }
-    // back to the checkpointed state, with the fastest setting
-    if ((rc = icempm_batch_restore(b))) return rc;
-    BCHK(b, hipMemcpy(b->stats, stats0, sizeof(stats0), hipMemcpyHostToDevice), "autotune");
-    BCHK(b, hipMemcpy(b->err, &err0, sizeof(int), hipMemcpyHostToDevice), "autotune");
-    b->resortEvery = bestResort;
-    if (bestChunk == b->chunkPoints) return ICEMPM_OK;   // the restored table is the one
-    b->chunkPoints = bestChunk;
-    return rebuild_chunks(b);
+        // back to the checkpointed state, with the fastest setting
+        if ((rc = icempm_batch_restore(b))) return rc;
+        BCHK(b, hipMemcpy(b->stats, stats0, sizeof(stats0), hipMemcpyHostToDevice), "autotune");
+        BCHK(b, hipMemcpy(b->err, &err0, sizeof(int), hipMemcpyHostToDevice), "autotune");
+        b->resortEvery = bestResort;
+        if (bestChunk == b->chunkPoints) return ICEMPM_OK;   // the restored table is the one
+        b->chunkPoints = bestChunk;
+        return rebuild_chunks(b);
+    });
 }
 
 int icempm_batch_map_stats(icempm_batch *b, unsigned long long *out, int reset) {
-    if (!b || !out) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "map_stats: batch not finalized");
-    BCHK(b, hipStreamSynchronize(b->stream), "map_stats");
-    unsigned long long all[kMapStats];
-    BCHK(b, hipMemcpy(all, b->stats, sizeof(all), hipMemcpyDeviceToHost), "map_stats");
-    std::memcpy(out, all, 3 * sizeof(unsigned long long));
-    if (reset) {
-        const unsigned long long z[kMapStats] = {};
-        BCHK(b, hipMemcpy(b->stats, z, sizeof(z), hipMemcpyHostToDevice), "map_stats");
-    }
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !out) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "map_stats: batch not finalized");
+        BCHK(b, hipStreamSynchronize(b->stream), "map_stats");
+        unsigned long long all[kMapStats];
+        BCHK(b, hipMemcpy(all, b->stats, sizeof(all), hipMemcpyDeviceToHost), "map_stats");
+        std::memcpy(out, all, 3 * sizeof(unsigned long long));
+        if (reset) {
+            const unsigned long long z[kMapStats] = {};
+            BCHK(b, hipMemcpy(b->stats, z, sizeof(z), hipMemcpyHostToDevice), "map_stats");
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_map_branch_stats(icempm_batch *b, unsigned long long *out, int reset) {
-    if (!b || !out) return ICEMPM_E_ARG;
-    if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "map_branch_stats: batch not finalized");
-    BCHK(b, hipStreamSynchronize(b->stream), "map_branch_stats");
-    unsigned long long all[kMapStats];
-    BCHK(b, hipMemcpy(all, b->stats, sizeof(all), hipMemcpyDeviceToHost), "map_branch_stats");
-    std::memcpy(out, all + 3, 4 * sizeof(unsigned long long));
-    if (reset) {
-        const unsigned long long z[kMapStats] = {};
-        BCHK(b, hipMemcpy(b->stats, z, sizeof(z), hipMemcpyHostToDevice), "map_branch_stats");
-    }
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || !out) return ICEMPM_E_ARG;
+        if (!b->finalized) return bfail(b, ICEMPM_E_STATE, "map_branch_stats: batch not finalized");
+        BCHK(b, hipStreamSynchronize(b->stream), "map_branch_stats");
+        unsigned long long all[kMapStats];
+        BCHK(b, hipMemcpy(all, b->stats, sizeof(all), hipMemcpyDeviceToHost), "map_branch_stats");
+        std::memcpy(out, all + 3, 4 * sizeof(unsigned long long));
+        if (reset) {
+            const unsigned long long z[kMapStats] = {};
+            BCHK(b, hipMemcpy(b->stats, z, sizeof(z), hipMemcpyHostToDevice), "map_branch_stats");
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_set_map_branches(icempm_batch *b, int mask) {
-    if (!b || mask < 0 || mask > (kMapTension | kMapShear)) return ICEMPM_E_ARG;
-    if (mask != b->mapBranches) {
-        BCHK(b, hipStreamSynchronize(b->stream), "set_map_branches");
-        drop_graph(b);   // the captured launches hold the mask (and the map kernel or not)
-        b->mapBranches = mask;
-    }
-    return ICEMPM_OK;
+    return host_guard(b ? &b->lastError : nullptr, [&]() -> int {
+        if (!b || mask < 0 || mask > (kMapTension | kMapShear)) return ICEMPM_E_ARG;
+        if (mask != b->mapBranches) {
+            BCHK(b, hipStreamSynchronize(b->stream), "set_map_branches");
+            drop_graph(b);   // the captured launches hold the mask (and the map kernel or not)
+            b->mapBranches = mask;
+        }
+        return ICEMPM_OK;
+    });
 }
 
 int icempm_batch_get_map_branches(const icempm_batch *b) { return b ? b->mapBranches : -1; }

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "host_guard.h"
 #include "mpm_device.h"
 
 namespace icempm {

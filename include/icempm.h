@@ -11,8 +11,10 @@
  *
  * Conventions
  *   - Every function returns 0 (ICEMPM_OK) or a nonzero status; the text of
- *     the last failure is returned by icempm_last_error().  The C++ surface
- *     rethrows with the reference's std::runtime_error tags.
+ *     the last failure is returned by icempm_last_error().  No C++ exception
+ *     leaves a function of this interface (a failed host allocation returns
+ *     ICEMPM_E_HOST).  The C++ surface rethrows with the reference's
+ *     std::runtime_error tags.
  *   - All device work of a context is queued on one HIP stream, in call
  *     order.  Calls on one context must not overlap (the reference calls
  *     every method from one simulation thread).
@@ -41,7 +43,9 @@ enum {
     ICEMPM_E_ARG = 2,      /* bad argument (null, size mismatch, out of range) */
     ICEMPM_E_STATE = 3,    /* call out of order (e.g. step before allocate) */
     ICEMPM_E_NODEV = 4,    /* no HIP device */
-    ICEMPM_E_IO = 5        /* file input/output (libicempm_h5.so) */
+    ICEMPM_E_IO = 5,       /* file input/output (libicempm_h5.so) */
+    ICEMPM_E_HOST = 6      /* host memory (or a thread) could not be allocated; the handle's
+                              state is then unspecified: destroy it */
 };
 
 /* P2G summation modes (icempm_set_mode).  FAST accumulates in LDS and then

@@ -124,3 +124,20 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
 
 
 import numpy as np  # noqa: E402
+
+
+def test_host_guard_turns_exceptions_into_status(tmp_path):
+    """No C++ exception leaves a C entry point: the guard every entry point of
+    libicempm_hip.so runs its body in returns ICEMPM_E_HOST with the text."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not found")
+    exe = str(tmp_path / "host_guard_check")
+    subprocess.run([gxx, "-std=c++17", "-O1", "-Wall", f"-I{os.path.join(ROOT, 'icempm_amd', 'csrc')}",
+                    os.path.join(ROOT, "tests", "cpp", "host_guard_check.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok ") == 6, r.stdout
+    assert _lib.ICEMPM_E_HOST == 6
