@@ -45,7 +45,12 @@ int icempm_simparams_parse_file(void *prms, const char *path, char *raw_points, 
 int icempm_model_create(icempm_model **out) {
     if (!out) return ICEMPM_E_ARG;
     *out = nullptr;
-    icempm_model *m = new icempm_model();
+    icempm_model *m = nullptr;
+    try {
+        m = new icempm_model();
+    } catch (...) {   // no exception crosses the C interface
+        return ICEMPM_E_HOST;
+    }
     m->snapshot.model = &m->model;
     int rc = icempm_guarded(m, [&] { m->model.gpu.initialize(); });
     if (rc != ICEMPM_OK) {
