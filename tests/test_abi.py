@@ -141,3 +141,29 @@ def test_host_guard_turns_exceptions_into_status(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("ok ") == 6, r.stdout
     assert _lib.ICEMPM_E_HOST == 6
+
+
+def test_every_hip_entry_point_is_guarded():
+    """Each C entry point of libicempm_hip.so with a body of its own runs it
+    behind host_guard (one-line accessors excepted): a scan of the sources."""
+    unguarded = []
+    for src in ("api.hip", "batch.hip"):
+        lines = open(os.path.join(ROOT, "icempm_amd", "csrc", src)).read().split("\n")
+        in_c = False
+        for i, line in enumerate(lines):
+            if line.startswith('extern "C" {'):
+                in_c = True
+            elif line.startswith('}  // extern "C"'):
+                in_c = False
+            m = re.match(r"^int (icempm_\w+)\(", line)
+            if not (in_c and m) or line.rstrip().endswith("}"):
+                continue
+            j = i
+            while not lines[j].rstrip().endswith(") {"):
+                j += 1
+            first_param = "".join(lines[i:j + 1]).split("(", 1)[1].split(",")[0]
+            if first_param.startswith("const icempm_"):
+                continue   # read-only accessors
+            if not lines[j + 1].strip().startswith("return host_guard("):
+                unguarded.append(f"{src}:{i + 1} {m.group(1)}")
+    assert not unguarded, unguarded
